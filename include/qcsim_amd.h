@@ -1,0 +1,188 @@
+/*
+ * qcsim_amd.h - C-ABI of the MI355X (gfx950) statevector engine.
+ *
+ * This is the drop-in boundary under the C++ facade in include/QCSim/. Each
+ * entry point replaces one piece of the reference's QC::QubitRegister /
+ * QubitRegisterCalculator path (reference files cited per function). Only
+ * plain C types cross the boundary: no HIP, no torch, no Eigen.
+ *
+ * Conventions (same as the reference, QubitRegister.h:7, SimpleGates.h:5):
+ *   - qubit 0 is the least-significant bit of a basis-state index;
+ *   - amplitudes cross the boundary as interleaved (re, im) doubles;
+ *   - gate matrices are row-major 2^k x 2^k complex doubles (interleaved),
+ *     local bit 0 = `qubit`, bit 1 = `c1`, bit 2 = `c2`
+ *     (QubitRegister.h:444-471, SimpleGates.h:329,430).
+ *
+ * Every function returns QCS_OK (0) or an error code; qcs_last_error()
+ * gives a thread-local message for the last failure on the calling thread.
+ * Calls on different registers may run concurrently from different host
+ * threads (one HIP stream per register); calls on one register must be
+ * serialised by the caller, as with the reference's QubitRegister.
+ */
+#ifndef QCSIM_AMD_H
+#define QCSIM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define QCS_OK      0
+#define QCS_EINVAL  1 /* bad argument (CheckQubits, QubitRegister.h:677-690) */
+#define QCS_ENOMEM  2 /* device allocation failed                          */
+#define QCS_EDEVICE 3 /* HIP runtime / launch failure                      */
+
+/* ---- storage precision ----------------------------------------------- */
+#define QCS_COMPLEX128 0 /* std::complex<double>, the reference's only type */
+#define QCS_COMPLEX64  1 /* complex<float> storage, FP64 reductions          */
+
+/* ---- gate traits: the QuantumGate virtuals, SimpleGates.h:27-60 ------- */
+#define QCS_TRAIT_CONTROLLED   0x01u /* isControlled()                    */
+#define QCS_TRAIT_CTRL_Q1      0x02u /* isControlQubit(1) (3-qubit gates) */
+#define QCS_TRAIT_DIAGONAL     0x04u /* isDiagonal()                      */
+#define QCS_TRAIT_ANTIDIAGONAL 0x08u /* isAntidiagonal()                  */
+#define QCS_TRAIT_SWAP         0x10u /* isSwapGate()                      */
+#define QCS_TRAIT_ISWAP        0x20u /* IsISwapGate()                     */
+#define QCS_TRAIT_ISWAPDAG     0x40u /* IsISwapDagGate()                  */
+
+typedef struct qcs_register qcs_register;
+
+/* One gate of a circuit batch (qcs_apply_circuit). */
+typedef struct qcs_gate {
+    uint32_t k;        /* 1..3: reference ApplyGate; 4..5: dense block      */
+    uint32_t traits;   /* QCS_TRAIT_* (ignored for k > 3)                    */
+    uint32_t qubits[5];/* k<=3: qubit, c1, c2; k>3: local bit i -> qubits[i] */
+    uint32_t pad;
+    double   mat[2 * 32 * 32]; /* row-major interleaved, first 2*4^k used   */
+} qcs_gate;
+
+/* ---- lifecycle -------------------------------------------------------- */
+
+/* QubitRegister(N) ctor, QubitRegister.h:17-37: allocates 2^n amplitudes in
+ * HBM (in place, no results buffer) and sets |0...0>. */
+int qcs_create(uint32_t nqubits, int precision, qcs_register** out);
+int qcs_destroy(qcs_register* r);
+uint32_t qcs_get_nqubits(const qcs_register* r);  /* getNrQubits, :59 */
+int qcs_get_precision(const qcs_register* r);
+/* Device copy of the state (SaveState/RestoreState/Clone, :600-616,662-674).
+ * Registers must have equal size and precision. */
+int qcs_copy_state(qcs_register* dst, const qcs_register* src);
+/* O(1) exchange of the two device buffers (setRegisterStorageFastNoNormalize
+ * and RestoreStateDestructive swap semantics, :521-524, :611-616). */
+int qcs_swap_state(qcs_register* a, qcs_register* b);
+
+/* ---- state access (QubitRegister.h:62-130, 507-524) ------------------- */
+
+/* Upload / download all 2^n amplitudes (interleaved re,im doubles). */
+int qcs_set_state(qcs_register* r, const double* amps, uint64_t n_amps);
+int qcs_get_state(qcs_register* r, double* amps, uint64_t n_amps);
+/* getBasisStateAmplitude / setRawAmplitude: index out of range is a silent
+ * no-op / returns 0, as in the reference (:62-66, :112-117). */
+int qcs_get_amplitude(qcs_register* r, uint64_t idx, double* re_im);
+int qcs_set_amplitude(qcs_register* r, uint64_t idx, double re, double im);
+/* setToBasisState (:74-80), out-of-range index is a silent no-op. */
+int qcs_set_basis_state(qcs_register* r, uint64_t idx);
+/* setConstant: Clear() is (0,0), setToEqualSuperposition (:106-109) is
+ * (1/sqrt(2^n), 0). */
+int qcs_fill(qcs_register* r, double re, double im);
+/* psi *= (re + i im) for every amplitude. */
+int qcs_scale(qcs_register* r, double re, double im);
+
+/* ---- gates ------------------------------------------------------------ */
+
+/* QubitRegister::ApplyGate(gate, qubit, c1, c2), QubitRegister.h:434-486.
+ * k = gate.getQubitsNumber() in 1..3, traits = the gate's trait bits, mat =
+ * gate.getRawOperatorMatrix(). Dispatch follows the reference exactly
+ * (QubitRegisterCalculator.h:39-227, 599-639): the traits choose the kernel
+ * and the kernel reads only the matrix entries the reference reads. Qubit
+ * checks and error cases are those of CheckQubits (:677-690). */
+int qcs_apply_gate(qcs_register* r, uint32_t k, uint32_t traits,
+                   uint32_t qubit, uint32_t c1, uint32_t c2,
+                   const double* mat);
+
+/* Dense k-qubit block (1 <= k <= 5) on arbitrary distinct qubits; local bit
+ * i of the matrix index is qubits[i]. Produced by the fusion pass; for k<=3
+ * it is the reference's AppliedGate generic path (SimpleGates.h:439-578). */
+int qcs_apply_dense(qcs_register* r, uint32_t k, const uint32_t* qubits,
+                    const double* mat);
+
+/* A batch of gates in order; fuse_width 0/1 applies them one by one, 2..5
+ * runs the fusion pass first (blocks of at most fuse_width qubits). */
+int qcs_apply_circuit(qcs_register* r, const qcs_gate* gates, uint32_t n,
+                      uint32_t fuse_width);
+
+/* Diagonal operator on the whole register: psi[s] *= diag[s]
+ * (ApplyOperatorMatrix with a diagonal matrix, QubitRegister.h:499-505). */
+int qcs_apply_diagonal(qcs_register* r, const double* diag, uint64_t n);
+
+/* ---- QFT (QuantumFourierTransform.h:35-87, QubitsSwapper.h:23-33) ----- */
+
+/* QFT (inverse=0) or IQFT (inverse=1) on qubits [sq, eq]; eq is clamped as
+ * QuantumSubAlgorithmOnSubregister does (QuantumAlgorithm.h:228-229).
+ * Result equals the reference's H / controlled-phase / SWAP sequence. */
+int qcs_qft(qcs_register* r, uint32_t sq, uint32_t eq, int do_swap,
+            int inverse);
+
+/* ---- measurement (uniform u in [0,1) supplied by the caller; the
+ *      reference draws it from its mt19937_64 and uses prob = 1 - u) ---- */
+
+/* MeasureAll, QubitRegister.h:169-195: first i with prob <= cdf(i), default
+ * 2^n-1; collapses to |i>. */
+int qcs_measure_all(qcs_register* r, double u, uint64_t* outcome);
+/* Measure(first, second) / MeasureQubit (:198-224, Calc:948-1059,
+ * 1124-1225): returns the measured sub-register value and collapses. */
+int qcs_measure(qcs_register* r, uint32_t first, uint32_t second, double u,
+                uint64_t* outcome);
+/* MeasureNoCollapse() (:619-642, default 0) when all != 0, else the
+ * protected MeasureNoCollapse(first, second) (:705-713, Calc:1061-1086,
+ * 1227-1254). The state is left untouched. */
+int qcs_measure_no_collapse(qcs_register* r, int all, uint32_t first,
+                            uint32_t second, double u, uint64_t* outcome);
+/* RepeatedMeasure[(first, second)] sampling, QubitRegister.h:227-429: one
+ * outcome per supplied uniform, from the CDF truncated once it exceeds
+ * 1 - DBL_EPSILON (lower_bound semantics, so an outcome may equal the
+ * truncated length). With all != 0 outcomes are full basis-state indices;
+ * otherwise (state & mask) >> first. nshots == 1 follows the reference's
+ * single-shot shortcut (MeasureNoCollapse). */
+int qcs_sample(qcs_register* r, int all, uint32_t first, uint32_t second,
+               uint64_t nshots, const double* u, uint64_t* outcomes);
+
+/* GetQubitProbability(q), QubitRegister.h:592-598, Calc:1088-1122. FP64. */
+int qcs_qubit_probability(qcs_register* r, uint32_t q, double* p);
+/* sum |psi|^2 in FP64 (the square of Eigen's norm(), :126). */
+int qcs_norm2(qcs_register* r, double* n2);
+/* Normalize, :124-130 (no-op when the norm is below 1e-20). */
+int qcs_normalize(qcs_register* r);
+/* <a|b> = sum conj(a_s) b_s in FP64 (stateFidelity :527-534 and
+ * ExpectationValue :646-660). a and b may differ in precision. */
+int qcs_inner(qcs_register* a, qcs_register* b, double* re_im);
+/* max_s |a_s - b_s| and <a|b>, for error reports between precisions. */
+int qcs_compare(qcs_register* a, qcs_register* b, double* max_abs,
+                double* inner_re_im);
+/* sum_s |psi_s|^2 * diag_energy(s) for the Ising energy
+ * E(s) = -(sum_i h_i z_i + sum_(i,j) J_ij z_i z_j), z = +1 for bit 0
+ * (IsingModel.h:106-119, 203-206): the exact expectation that the
+ * reference estimates by sampling (IsingModel.h:569-578). FP64. */
+int qcs_ising_energy(qcs_register* r, uint32_t n_edges, const uint32_t* ei,
+                     const uint32_t* ej, const double* J, const double* h,
+                     double* energy);
+
+/* ---- stream / timing --------------------------------------------------- */
+int qcs_synchronize(qcs_register* r);
+/* HIP events on the register's stream around the enclosed calls. */
+int qcs_timer_begin(qcs_register* r);
+int qcs_timer_end(qcs_register* r, float* ms);
+/* Number of kernels launched on the register since creation. */
+uint64_t qcs_launch_count(const qcs_register* r);
+
+const char* qcs_last_error(void);
+const char* qcs_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QCSIM_AMD_H */

@@ -1,0 +1,122 @@
+// common.h - internal types of the gfx950 statevector engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "qcsim_amd.h"
+
+// Device-side complex types: amplitudes are interleaved (re, im) pairs,
+// 16 B (complex128) or 8 B (complex64), so one amplitude is one
+// global_load_dwordx4 / dwordx2.
+template <typename T> struct cvec;
+template <> struct cvec<double> { using type = double2; };
+template <> struct cvec<float> { using type = float2; };
+template <typename T> using cv = typename cvec<T>::type;
+
+struct hcplx {
+    double re, im;
+};
+
+// Gate matrices passed by value in kernel arguments (row-major).
+template <typename T, int D> struct MatD {
+    T re[D * D];
+    T im[D * D];
+};
+
+// Positions (ascending) at which zero bits are inserted into an orbit index
+// to obtain the index of the orbit's first amplitude.
+struct Ins {
+    int n;
+    uint8_t p[6];
+};
+
+__device__ __forceinline__ uint64_t insert_zeros(uint64_t o, const Ins& ins) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        if (i < ins.n) {
+            const uint64_t low = (1ull << ins.p[i]) - 1;
+            o = ((o & ~low) << 1) | (o & low);
+        }
+    }
+    return o;
+}
+
+// Contiguous run of `len` zero bits inserted at `pos` (range measurements).
+__device__ __forceinline__ uint64_t insert_run(uint64_t o, uint32_t pos, uint32_t len) {
+    const uint64_t low = (1ull << pos) - 1;
+    return ((o & ~low) << len) | (o & low);
+}
+
+template <typename T>
+__device__ __forceinline__ cv<T> cmul(T ar, T ai, cv<T> x) {
+    cv<T> r;
+    r.x = ar * x.x - ai * x.y;
+    r.y = ar * x.y + ai * x.x;
+    return r;
+}
+
+// Engine register: one HBM buffer of 2^n amplitudes, updated in place,
+// plus FP64 reduction scratch and its own HIP stream.
+struct qcs_register {
+    uint32_t n = 0;
+    uint64_t A = 0;
+    int prec = QCS_COMPLEX128;
+    int device = 0;
+    void* psi = nullptr;        // 2^n amplitudes
+    double* red = nullptr;      // reduction partials (kRedBlocks * 4 doubles)
+    double* chunk = nullptr;    // per-chunk CDF (A / kChunk doubles, >= 1)
+    uint64_t* dout = nullptr;   // search results / small device outputs
+    void* mat = nullptr;        // dense matrix staging (device)
+    size_t mat_bytes = 0;
+    double* hpin = nullptr;     // pinned host scalars
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    uint64_t launches = 0;
+
+    size_t amp_bytes() const { return prec == QCS_COMPLEX128 ? 16 : 8; }
+};
+
+namespace qcs {
+
+constexpr int kRedBlocks = 1024;     // fixed grid of reductions (deterministic)
+constexpr int kRedThreads = 256;
+constexpr uint32_t kChunkLog = 11;   // CDF chunk = 2048 amplitudes
+constexpr int kScanThreads = 256;
+
+int set_error(int code, const char* fmt, ...);
+int check_launch(qcs_register* r, const char* what);
+
+// gate_kernels.hip
+int k1q(qcs_register* r, int mode, uint32_t target, const uint32_t* ctrls, int nctrl,
+        const hcplx* m /* 4 */);
+int kphase(qcs_register* r, const uint32_t* bits, int nbits, hcplx ph);
+int kswap(qcs_register* r, uint32_t a, uint32_t b, const uint32_t* ctrls, int nctrl, hcplx ph,
+          bool phased);
+int k2q(qcs_register* r, uint32_t t0, uint32_t t1, const uint32_t* ctrls, int nctrl,
+        const hcplx* m /* 16 */);
+int k3q(qcs_register* r, uint32_t t0, uint32_t t1, uint32_t t2, const hcplx* m /* 64 */);
+int kdiag_full(qcs_register* r, const double* diag_dev);
+int kfill(qcs_register* r, double re, double im);
+int kscale(qcs_register* r, double re, double im);
+int kset_one(qcs_register* r, uint64_t idx, double re, double im);
+
+// dense_kernels.hip: dense k-qubit blocks (k <= 5) on arbitrary qubits
+int kdense(qcs_register* r, uint32_t k, const uint32_t* qubits, const double* mat_host);
+
+// reduce_kernels.hip (results in FP64)
+int rsum_sq(qcs_register* r, uint32_t first, uint32_t len, uint64_t val, double* out);
+int rinner(qcs_register* a, qcs_register* b, double* re_im, double* max_abs);
+int cdf_search(qcs_register* r, uint64_t nshots, const double* u_host, int strict,
+               uint64_t dflt, uint64_t* out_host, int last_in_chunk_fallback);
+int kcollapse(qcs_register* r, uint64_t mask, uint64_t val, double scale);
+int rising(qcs_register* r, uint32_t ne, const uint32_t* ei, const uint32_t* ej,
+           const double* J, const double* h, double* energy);
+
+// qft.hip
+int qft(qcs_register* r, uint32_t sq, uint32_t eq, int do_swap, int inverse);
+
+}  // namespace qcs

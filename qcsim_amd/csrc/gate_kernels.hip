@@ -1,0 +1,404 @@
+// gate_kernels.hip - in-place gate kernels for gfx950.
+//
+// Every kernel enumerates "orbits": the set of amplitudes a gate mixes.
+// Orbit o is expanded into its first amplitude index by inserting zero bits
+// at the positions of the gate's target and control qubits and OR-ing the
+// control mask; the other amplitudes of the orbit differ only in target bits.
+// A work-item owns a whole orbit, so updates are in place and race-free, and
+// control=0 amplitudes are never read or written (the reference copies them,
+// QubitRegisterCalculator.h:377-378,400-401). Consecutive lanes take
+// consecutive orbits, so for target qubits >= 6 each wave streams contiguous
+// 1 KiB (FP64) runs per orbit member; for low targets a wave's members
+// interleave inside one contiguous 2-4 KiB span, still whole cache lines.
+#include <algorithm>
+
+#include "common.h"
+
+namespace qcs {
+
+constexpr int kThreads = 256;
+
+static Ins make_ins(const uint32_t* pos, int n) {
+    Ins ins{};
+    uint32_t tmp[6];
+    for (int i = 0; i < n; ++i) tmp[i] = pos[i];
+    std::sort(tmp, tmp + n);
+    ins.n = n;
+    for (int i = 0; i < n; ++i) ins.p[i] = uint8_t(tmp[i]);
+    return ins;
+}
+
+static unsigned grid_for(uint64_t norb, int per_block) {
+    return unsigned((norb + per_block - 1) / per_block);
+}
+
+// ---------------------------------------------------------------------------
+// 1-qubit update on the orbit {i0, i0|bit} (plus controls in cmask).
+// MODE 0: generic 2x2 (K3, K8, K13); 1: antidiagonal (K2, K7, K12);
+// 2: diagonal with both entries (K1, K6, K11).
+template <typename T, int MODE, int P>
+__global__ __launch_bounds__(kThreads) void k_1q(cv<T>* __restrict__ psi, uint64_t norb, Ins ins,
+                                                 uint64_t cmask, uint64_t bit, MatD<T, 2> m) {
+    const uint64_t base = uint64_t(blockIdx.x) * (kThreads * P) + threadIdx.x;
+    cv<T> a[P], b[P];
+    uint64_t i0[P];
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        const uint64_t o = base + uint64_t(j) * kThreads;
+        i0[j] = insert_zeros(o, ins) | cmask;
+        if (o < norb) {
+            a[j] = psi[i0[j]];
+            b[j] = psi[i0[j] | bit];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        const uint64_t o = base + uint64_t(j) * kThreads;
+        if (o >= norb) continue;
+        cv<T> u, v;
+        if (MODE == 0) {
+            u.x = m.re[0] * a[j].x - m.im[0] * a[j].y + m.re[1] * b[j].x - m.im[1] * b[j].y;
+            u.y = m.re[0] * a[j].y + m.im[0] * a[j].x + m.re[1] * b[j].y + m.im[1] * b[j].x;
+            v.x = m.re[2] * a[j].x - m.im[2] * a[j].y + m.re[3] * b[j].x - m.im[3] * b[j].y;
+            v.y = m.re[2] * a[j].y + m.im[2] * a[j].x + m.re[3] * b[j].y + m.im[3] * b[j].x;
+        } else if (MODE == 1) {
+            u = cmul<T>(m.re[1], m.im[1], b[j]);
+            v = cmul<T>(m.re[2], m.im[2], a[j]);
+        } else {
+            u = cmul<T>(m.re[0], m.im[0], a[j]);
+            v = cmul<T>(m.re[3], m.im[3], b[j]);
+        }
+        psi[i0[j]] = u;
+        psi[i0[j] | bit] = v;
+    }
+}
+
+// Phase on the single amplitude of each orbit whose listed bits are all 1:
+// P/S/T/Z (m00 == 1), CZ/CP/CRz-with-unit-m22 (K6), CCZ (K11).
+template <typename T, int P>
+__global__ __launch_bounds__(kThreads) void k_phase(cv<T>* __restrict__ psi, uint64_t norb, Ins ins,
+                                                    uint64_t cmask, T pr, T pi) {
+    const uint64_t base = uint64_t(blockIdx.x) * (kThreads * P) + threadIdx.x;
+    cv<T> a[P];
+    uint64_t i0[P];
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        const uint64_t o = base + uint64_t(j) * kThreads;
+        i0[j] = insert_zeros(o, ins) | cmask;
+        if (o < norb) a[j] = psi[i0[j]];
+    }
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        const uint64_t o = base + uint64_t(j) * kThreads;
+        if (o < norb) psi[i0[j]] = cmul<T>(pr, pi, a[j]);
+    }
+}
+
+// Exchange psi[i|ba] <-> psi[i|bb], optionally times a phase (SWAP K4,
+// iSWAP/iSWAP-dagger K5, Fredkin K10 with its control in cmask).
+template <typename T, bool PHASED, int P>
+__global__ __launch_bounds__(kThreads) void k_swap(cv<T>* __restrict__ psi, uint64_t norb, Ins ins,
+                                                   uint64_t cmask, uint64_t ba, uint64_t bb, T pr,
+                                                   T pi) {
+    const uint64_t base = uint64_t(blockIdx.x) * (kThreads * P) + threadIdx.x;
+    cv<T> a[P], b[P];
+    uint64_t i0[P];
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        const uint64_t o = base + uint64_t(j) * kThreads;
+        i0[j] = insert_zeros(o, ins) | cmask;
+        if (o < norb) {
+            a[j] = psi[i0[j] | ba];
+            b[j] = psi[i0[j] | bb];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        const uint64_t o = base + uint64_t(j) * kThreads;
+        if (o >= norb) continue;
+        if (PHASED) {
+            psi[i0[j] | ba] = cmul<T>(pr, pi, b[j]);
+            psi[i0[j] | bb] = cmul<T>(pr, pi, a[j]);
+        } else {
+            psi[i0[j] | ba] = b[j];
+            psi[i0[j] | bb] = a[j];
+        }
+    }
+}
+
+// Generic 4x4 on target bits (b0 = local bit 0, b1 = local bit 1), with
+// optional controls: K9 and K14.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_2q(cv<T>* __restrict__ psi, uint64_t norb, Ins ins,
+                                                 uint64_t cmask, uint64_t b0, uint64_t b1,
+                                                 MatD<T, 4> m) {
+    const uint64_t o = uint64_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (o >= norb) return;
+    const uint64_t i0 = insert_zeros(o, ins) | cmask;
+    uint64_t idx[4] = {i0, i0 | b0, i0 | b1, i0 | b0 | b1};
+    cv<T> x[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) x[c] = psi[idx[c]];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        T yr = 0, yi = 0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            yr += m.re[r * 4 + c] * x[c].x - m.im[r * 4 + c] * x[c].y;
+            yi += m.re[r * 4 + c] * x[c].y + m.im[r * 4 + c] * x[c].x;
+        }
+        cv<T> y;
+        y.x = yr;
+        y.y = yi;
+        psi[idx[r]] = y;
+    }
+}
+
+// Generic 8x8 (K15): local bits (b0, b1, b2) = (qubit, c1, c2).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_3q(cv<T>* __restrict__ psi, uint64_t norb, Ins ins,
+                                                 uint64_t b0, uint64_t b1, uint64_t b2,
+                                                 MatD<T, 8> m) {
+    const uint64_t o = uint64_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (o >= norb) return;
+    const uint64_t i0 = insert_zeros(o, ins);
+    uint64_t idx[8];
+    cv<T> x[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        idx[c] = i0 | ((c & 1) ? b0 : 0) | ((c & 2) ? b1 : 0) | ((c & 4) ? b2 : 0);
+        x[c] = psi[idx[c]];
+    }
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        T yr = 0, yi = 0;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            yr += m.re[r * 8 + c] * x[c].x - m.im[r * 8 + c] * x[c].y;
+            yi += m.re[r * 8 + c] * x[c].y + m.im[r * 8 + c] * x[c].x;
+        }
+        cv<T> y;
+        y.x = yr;
+        y.y = yi;
+        psi[idx[r]] = y;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_fill(cv<T>* __restrict__ psi, uint64_t A, T re, T im) {
+    const uint64_t stride = uint64_t(gridDim.x) * kThreads;
+    cv<T> v;
+    v.x = re;
+    v.y = im;
+    for (uint64_t i = uint64_t(blockIdx.x) * kThreads + threadIdx.x; i < A; i += stride) psi[i] = v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_scale(cv<T>* __restrict__ psi, uint64_t A, T re, T im) {
+    const uint64_t stride = uint64_t(gridDim.x) * kThreads;
+    for (uint64_t i = uint64_t(blockIdx.x) * kThreads + threadIdx.x; i < A; i += stride)
+        psi[i] = cmul<T>(re, im, psi[i]);
+}
+
+template <typename T>
+__global__ void k_set_one(cv<T>* __restrict__ psi, uint64_t idx, T re, T im) {
+    cv<T> v;
+    v.x = re;
+    v.y = im;
+    psi[idx] = v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_diag_full(cv<T>* __restrict__ psi, uint64_t A,
+                                                        const double2* __restrict__ d) {
+    const uint64_t stride = uint64_t(gridDim.x) * kThreads;
+    for (uint64_t i = uint64_t(blockIdx.x) * kThreads + threadIdx.x; i < A; i += stride) {
+        const double2 f = d[i];
+        psi[i] = cmul<T>(T(f.x), T(f.y), psi[i]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers.
+
+static unsigned stream_grid(uint64_t A) {
+    const uint64_t per = uint64_t(kThreads) * 8;
+    uint64_t g = (A + per - 1) / per;
+    if (g > 65536) g = 65536;
+    if (g < 1) g = 1;
+    return unsigned(g);
+}
+
+template <typename T>
+static int k1q_t(qcs_register* r, int mode, uint32_t target, const uint32_t* ctrls, int nctrl,
+                 const hcplx* m) {
+    uint32_t pos[6];
+    uint64_t cmask = 0;
+    pos[0] = target;
+    for (int i = 0; i < nctrl; ++i) {
+        pos[1 + i] = ctrls[i];
+        cmask |= 1ull << ctrls[i];
+    }
+    const Ins ins = make_ins(pos, 1 + nctrl);
+    const uint64_t norb = r->A >> (1 + nctrl);
+    MatD<T, 2> mm;
+    for (int i = 0; i < 4; ++i) {
+        mm.re[i] = T(m[i].re);
+        mm.im[i] = T(m[i].im);
+    }
+    constexpr int P = 4;
+    const unsigned g = grid_for(norb, kThreads * P);
+    auto* psi = static_cast<cv<T>*>(r->psi);
+    const uint64_t bit = 1ull << target;
+    if (mode == 0)
+        hipLaunchKernelGGL((k_1q<T, 0, P>), dim3(g), dim3(kThreads), 0, r->stream, psi, norb, ins, cmask, bit, mm);
+    else if (mode == 1)
+        hipLaunchKernelGGL((k_1q<T, 1, P>), dim3(g), dim3(kThreads), 0, r->stream, psi, norb, ins, cmask, bit, mm);
+    else
+        hipLaunchKernelGGL((k_1q<T, 2, P>), dim3(g), dim3(kThreads), 0, r->stream, psi, norb, ins, cmask, bit, mm);
+    return check_launch(r, "k_1q");
+}
+
+int k1q(qcs_register* r, int mode, uint32_t target, const uint32_t* ctrls, int nctrl,
+        const hcplx* m) {
+    return r->prec == QCS_COMPLEX128 ? k1q_t<double>(r, mode, target, ctrls, nctrl, m)
+                                     : k1q_t<float>(r, mode, target, ctrls, nctrl, m);
+}
+
+template <typename T>
+static int kphase_t(qcs_register* r, const uint32_t* bits, int nbits, hcplx ph) {
+    uint64_t cmask = 0;
+    for (int i = 0; i < nbits; ++i) cmask |= 1ull << bits[i];
+    const Ins ins = make_ins(bits, nbits);
+    const uint64_t norb = r->A >> nbits;
+    constexpr int P = 4;
+    const unsigned g = grid_for(norb, kThreads * P);
+    hipLaunchKernelGGL((k_phase<T, P>), dim3(g), dim3(kThreads), 0, r->stream,
+                       static_cast<cv<T>*>(r->psi), norb, ins, cmask, T(ph.re), T(ph.im));
+    return check_launch(r, "k_phase");
+}
+
+int kphase(qcs_register* r, const uint32_t* bits, int nbits, hcplx ph) {
+    return r->prec == QCS_COMPLEX128 ? kphase_t<double>(r, bits, nbits, ph)
+                                     : kphase_t<float>(r, bits, nbits, ph);
+}
+
+template <typename T>
+static int kswap_t(qcs_register* r, uint32_t a, uint32_t b, const uint32_t* ctrls, int nctrl,
+                   hcplx ph, bool phased) {
+    uint32_t pos[6] = {a, b};
+    uint64_t cmask = 0;
+    for (int i = 0; i < nctrl; ++i) {
+        pos[2 + i] = ctrls[i];
+        cmask |= 1ull << ctrls[i];
+    }
+    const Ins ins = make_ins(pos, 2 + nctrl);
+    const uint64_t norb = r->A >> (2 + nctrl);
+    constexpr int P = 2;
+    const unsigned g = grid_for(norb, kThreads * P);
+    auto* psi = static_cast<cv<T>*>(r->psi);
+    if (phased)
+        hipLaunchKernelGGL((k_swap<T, true, P>), dim3(g), dim3(kThreads), 0, r->stream, psi, norb, ins,
+                           cmask, 1ull << a, 1ull << b, T(ph.re), T(ph.im));
+    else
+        hipLaunchKernelGGL((k_swap<T, false, P>), dim3(g), dim3(kThreads), 0, r->stream, psi, norb, ins,
+                           cmask, 1ull << a, 1ull << b, T(0), T(0));
+    return check_launch(r, "k_swap");
+}
+
+int kswap(qcs_register* r, uint32_t a, uint32_t b, const uint32_t* ctrls, int nctrl, hcplx ph,
+          bool phased) {
+    return r->prec == QCS_COMPLEX128 ? kswap_t<double>(r, a, b, ctrls, nctrl, ph, phased)
+                                     : kswap_t<float>(r, a, b, ctrls, nctrl, ph, phased);
+}
+
+template <typename T>
+static int k2q_t(qcs_register* r, uint32_t t0, uint32_t t1, const uint32_t* ctrls, int nctrl,
+                 const hcplx* m) {
+    uint32_t pos[6] = {t0, t1};
+    uint64_t cmask = 0;
+    for (int i = 0; i < nctrl; ++i) {
+        pos[2 + i] = ctrls[i];
+        cmask |= 1ull << ctrls[i];
+    }
+    const Ins ins = make_ins(pos, 2 + nctrl);
+    const uint64_t norb = r->A >> (2 + nctrl);
+    MatD<T, 4> mm;
+    for (int i = 0; i < 16; ++i) {
+        mm.re[i] = T(m[i].re);
+        mm.im[i] = T(m[i].im);
+    }
+    hipLaunchKernelGGL((k_2q<T>), dim3(grid_for(norb, kThreads)), dim3(kThreads), 0, r->stream,
+                       static_cast<cv<T>*>(r->psi), norb, ins, cmask, 1ull << t0, 1ull << t1, mm);
+    return check_launch(r, "k_2q");
+}
+
+int k2q(qcs_register* r, uint32_t t0, uint32_t t1, const uint32_t* ctrls, int nctrl,
+        const hcplx* m) {
+    return r->prec == QCS_COMPLEX128 ? k2q_t<double>(r, t0, t1, ctrls, nctrl, m)
+                                     : k2q_t<float>(r, t0, t1, ctrls, nctrl, m);
+}
+
+template <typename T>
+static int k3q_t(qcs_register* r, uint32_t t0, uint32_t t1, uint32_t t2, const hcplx* m) {
+    uint32_t pos[3] = {t0, t1, t2};
+    const Ins ins = make_ins(pos, 3);
+    const uint64_t norb = r->A >> 3;
+    MatD<T, 8> mm;
+    for (int i = 0; i < 64; ++i) {
+        mm.re[i] = T(m[i].re);
+        mm.im[i] = T(m[i].im);
+    }
+    hipLaunchKernelGGL((k_3q<T>), dim3(grid_for(norb, kThreads)), dim3(kThreads), 0, r->stream,
+                       static_cast<cv<T>*>(r->psi), norb, ins, 1ull << t0, 1ull << t1, 1ull << t2, mm);
+    return check_launch(r, "k_3q");
+}
+
+int k3q(qcs_register* r, uint32_t t0, uint32_t t1, uint32_t t2, const hcplx* m) {
+    return r->prec == QCS_COMPLEX128 ? k3q_t<double>(r, t0, t1, t2, m) : k3q_t<float>(r, t0, t1, t2, m);
+}
+
+int kfill(qcs_register* r, double re, double im) {
+    const unsigned g = stream_grid(r->A);
+    if (r->prec == QCS_COMPLEX128)
+        hipLaunchKernelGGL((k_fill<double>), dim3(g), dim3(kThreads), 0, r->stream,
+                           static_cast<double2*>(r->psi), r->A, re, im);
+    else
+        hipLaunchKernelGGL((k_fill<float>), dim3(g), dim3(kThreads), 0, r->stream,
+                           static_cast<float2*>(r->psi), r->A, float(re), float(im));
+    return check_launch(r, "k_fill");
+}
+
+int kscale(qcs_register* r, double re, double im) {
+    const unsigned g = stream_grid(r->A);
+    if (r->prec == QCS_COMPLEX128)
+        hipLaunchKernelGGL((k_scale<double>), dim3(g), dim3(kThreads), 0, r->stream,
+                           static_cast<double2*>(r->psi), r->A, re, im);
+    else
+        hipLaunchKernelGGL((k_scale<float>), dim3(g), dim3(kThreads), 0, r->stream,
+                           static_cast<float2*>(r->psi), r->A, float(re), float(im));
+    return check_launch(r, "k_scale");
+}
+
+int kset_one(qcs_register* r, uint64_t idx, double re, double im) {
+    if (r->prec == QCS_COMPLEX128)
+        hipLaunchKernelGGL((k_set_one<double>), dim3(1), dim3(1), 0, r->stream,
+                           static_cast<double2*>(r->psi), idx, re, im);
+    else
+        hipLaunchKernelGGL((k_set_one<float>), dim3(1), dim3(1), 0, r->stream,
+                           static_cast<float2*>(r->psi), idx, float(re), float(im));
+    return check_launch(r, "k_set_one");
+}
+
+int kdiag_full(qcs_register* r, const double* diag_dev) {
+    const unsigned g = stream_grid(r->A);
+    if (r->prec == QCS_COMPLEX128)
+        hipLaunchKernelGGL((k_diag_full<double>), dim3(g), dim3(kThreads), 0, r->stream,
+                           static_cast<double2*>(r->psi), r->A, reinterpret_cast<const double2*>(diag_dev));
+    else
+        hipLaunchKernelGGL((k_diag_full<float>), dim3(g), dim3(kThreads), 0, r->stream,
+                           static_cast<float2*>(r->psi), r->A, reinterpret_cast<const double2*>(diag_dev));
+    return check_launch(r, "k_diag_full");
+}
+
+}  // namespace qcs

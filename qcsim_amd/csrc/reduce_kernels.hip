@@ -1,0 +1,480 @@
+// reduce_kernels.hip - FP64 reductions, CDF search and collapse.
+//
+// All sums are accumulated in FP64 whatever the storage precision, with a
+// fixed grid and a fixed combination order, so results are deterministic
+// run to run. The CDF search that the reference runs as one serial FP64 scan
+// (QubitRegister.h:171-190; QubitRegisterCalculator.h:955-970) is split into
+//   1. per-chunk inclusive scans of |psi|^2 (2048 amplitudes per block),
+//   2. one-block scan of the chunk totals into a chunk-level CDF,
+//   3. per shot: binary search of the chunk CDF, then the chunk's local scan
+//      is recomputed with the same code (bit-identical) to find the first
+//      index with prob <= cdf (or prob < cdf for the truncation threshold).
+// GPU and CPU prefix sums differ only by rounding; the tests report the
+// distance of each uniform to the nearest CDF boundary.
+#include <cfloat>
+#include <vector>
+
+#include "common.h"
+
+namespace qcs {
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Block (256 threads = 4 waves) sum; result valid in thread 0.
+__device__ __forceinline__ double block_sum256(double v, double* lds4) {
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) lds4[w] = v;
+    __syncthreads();
+    double t = 0;
+    if (threadIdx.x == 0) t = ((lds4[0] + lds4[1]) + lds4[2]) + lds4[3];
+    __syncthreads();
+    return t;
+}
+
+template <typename T>
+__device__ __forceinline__ double norm2(cv<T> a) {
+    const double x = double(a.x), y = double(a.y);
+    return x * x + y * y;
+}
+
+// Sum of |psi_s|^2 over s = insert_run(o, first, len) | val.
+template <typename T>
+__global__ __launch_bounds__(kRedThreads) void k_sum_sq(const cv<T>* __restrict__ psi, uint64_t norb,
+                                                        uint32_t first, uint32_t len, uint64_t val,
+                                                        double* __restrict__ partial) {
+    __shared__ double lds[4];
+    double acc = 0;
+    const uint64_t stride = uint64_t(gridDim.x) * kRedThreads;
+    for (uint64_t o = uint64_t(blockIdx.x) * kRedThreads + threadIdx.x; o < norb; o += stride)
+        acc += norm2<T>(psi[insert_run(o, first, len) | val]);
+    const double t = block_sum256(acc, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// Fixed-order sum of `n` partials (n <= kRedBlocks) by one block.
+__global__ __launch_bounds__(kRedThreads) void k_final_sum(const double* __restrict__ partial, int n,
+                                                           int stride_elems, double* __restrict__ out) {
+    __shared__ double lds[4];
+    for (int comp = 0; comp < stride_elems; ++comp) {
+        double acc = 0;
+        for (int i = threadIdx.x; i < n; i += kRedThreads) acc += partial[i * stride_elems + comp];
+        const double t = block_sum256(acc, lds);
+        if (threadIdx.x == 0) out[comp] = t;
+    }
+}
+
+template <typename Ta, typename Tb>
+__global__ __launch_bounds__(kRedThreads) void k_inner(const cv<Ta>* __restrict__ a, const cv<Tb>* __restrict__ b,
+                                                       uint64_t A, double* __restrict__ partial) {
+    __shared__ double lds[4];
+    double re = 0, im = 0, mx = 0;
+    const uint64_t stride = uint64_t(gridDim.x) * kRedThreads;
+    for (uint64_t s = uint64_t(blockIdx.x) * kRedThreads + threadIdx.x; s < A; s += stride) {
+        const cv<Ta> x = a[s];
+        const cv<Tb> y = b[s];
+        const double xr = x.x, xi = x.y, yr = y.x, yi = y.y;
+        re += xr * yr + xi * yi;  // conj(x) * y
+        im += xr * yi - xi * yr;
+        const double dr = xr - yr, di = xi - yi;
+        mx = fmax(mx, sqrt(dr * dr + di * di));
+    }
+    const double tr = block_sum256(re, lds);
+    const double ti = block_sum256(im, lds);
+    double m = wave_max(mx);
+    __shared__ double lm[4];
+    if ((threadIdx.x & 63) == 0) lm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x * 3 + 0] = tr;
+        partial[blockIdx.x * 3 + 1] = ti;
+        partial[blockIdx.x * 3 + 2] = fmax(fmax(lm[0], lm[1]), fmax(lm[2], lm[3]));
+    }
+}
+
+__global__ void k_final_max(const double* __restrict__ partial, int n, double* __restrict__ out) {
+    if (threadIdx.x != 0) return;
+    double m = 0;
+    for (int i = 0; i < n; ++i) m = fmax(m, partial[i * 3 + 2]);
+    out[2] = m;
+}
+
+// ---------------------------------------------------------------------------
+// CDF.
+
+struct ChunkGeom {
+    uint64_t clen;  // amplitudes per chunk
+    uint32_t per;   // amplitudes per thread (clen / 256, at least 1)
+};
+
+// Inclusive scan of |psi|^2 over one chunk. Thread t owns elements
+// [t*per, (t+1)*per); returns the exclusive prefix of thread t within the
+// chunk and the chunk total (to all threads). Identical code runs in the
+// chunk pass and in the search pass, so both see bit-identical values.
+template <typename T>
+__device__ __forceinline__ double chunk_thread_prefix(const cv<T>* __restrict__ psi, uint64_t cbase,
+                                                      ChunkGeom g, double* lds, double* total) {
+    double mine = 0;
+    const uint64_t beg = uint64_t(threadIdx.x) * g.per;
+    for (uint32_t i = 0; i < g.per; ++i) {
+        const uint64_t e = beg + i;
+        if (e < g.clen) mine += norm2<T>(psi[cbase + e]);
+    }
+    // inclusive wave scan
+    double v = mine;
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const double n = __shfl_up(v, o, 64);
+        if (lane >= o) v += n;
+    }
+    double excl = __shfl_up(v, 1, 64);
+    if (lane == 0) excl = 0;
+    const int w = threadIdx.x >> 6;
+    if (lane == 63) lds[w] = v;
+    __syncthreads();
+    double wpre = 0;
+    for (int k = 0; k < w; ++k) wpre += lds[k];
+    *total = ((lds[0] + lds[1]) + lds[2]) + lds[3];
+    __syncthreads();
+    return wpre + excl;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kScanThreads) void k_chunk_sums(const cv<T>* __restrict__ psi, ChunkGeom g,
+                                                             double* __restrict__ chunk) {
+    __shared__ double lds[4];
+    double total;
+    chunk_thread_prefix<T>(psi, uint64_t(blockIdx.x) * g.clen, g, lds, &total);
+    if (threadIdx.x == 0) chunk[blockIdx.x] = total;
+}
+
+// In-place inclusive scan of chunk totals by one block of 1024 threads.
+__global__ __launch_bounds__(1024) void k_scan_chunks(double* __restrict__ chunk, uint64_t nchunks) {
+    __shared__ double lds[16];
+    const uint64_t per = (nchunks + 1023) / 1024;
+    const uint64_t beg = uint64_t(threadIdx.x) * per;
+    double run = 0;
+    for (uint64_t i = 0; i < per; ++i) {
+        const uint64_t e = beg + i;
+        if (e < nchunks) {
+            run += chunk[e];
+            chunk[e] = run;
+        }
+    }
+    double v = run;
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const double n = __shfl_up(v, o, 64);
+        if (lane >= o) v += n;
+    }
+    double off = __shfl_up(v, 1, 64);
+    if (lane == 0) off = 0;
+    const int w = threadIdx.x >> 6;
+    if (lane == 63) lds[w] = v;
+    __syncthreads();
+    for (int k = 0; k < w; ++k) off += lds[k];
+    for (uint64_t i = 0; i < per; ++i) {
+        const uint64_t e = beg + i;
+        if (e < nchunks) chunk[e] += off;
+    }
+}
+
+// One block per shot. out[s] = first index i with prob <= cdf(i) (strict:
+// prob < cdf(i)), or `dflt` when the total stays below prob.
+template <typename T>
+__global__ __launch_bounds__(kScanThreads) void k_search(const cv<T>* __restrict__ psi, ChunkGeom g,
+                                                         const double* __restrict__ chunk, uint64_t nchunks,
+                                                         const double* __restrict__ probs, int strict,
+                                                         uint64_t dflt, uint64_t* __restrict__ out) {
+    __shared__ double lds[4];
+    __shared__ uint64_t sc;
+    __shared__ unsigned long long found;
+    const double prob = probs[blockIdx.x];
+    if (threadIdx.x == 0) {
+        // first c with chunk[c] >= prob (strict: > prob)
+        uint64_t lo = 0, hi = nchunks;
+        while (lo < hi) {
+            const uint64_t mid = (lo + hi) >> 1;
+            const double v = chunk[mid];
+            const bool ok = strict ? (v > prob) : (v >= prob);
+            if (ok) hi = mid;
+            else lo = mid + 1;
+        }
+        sc = lo;
+        found = ~0ull;
+    }
+    __syncthreads();
+    const uint64_t c = sc;
+    if (c >= nchunks) {
+        if (threadIdx.x == 0) out[blockIdx.x] = dflt;
+        return;
+    }
+    double total;
+    const uint64_t cbase = c * g.clen;
+    const double pre = chunk_thread_prefix<T>(psi, cbase, g, lds, &total);
+    double acc = (c > 0 ? chunk[c - 1] : 0.0) + pre;
+    const uint64_t beg = uint64_t(threadIdx.x) * g.per;
+    double run = 0;
+    for (uint32_t i = 0; i < g.per; ++i) {
+        const uint64_t e = beg + i;
+        if (e >= g.clen) break;
+        run += norm2<T>(psi[cbase + e]);
+        const double v = acc + run;
+        const bool ok = strict ? (v > prob) : (v >= prob);
+        if (ok) {
+            atomicMin(&found, (unsigned long long)(cbase + e));
+            break;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long f = found;
+        out[blockIdx.x] = (f == ~0ull) ? (cbase + g.clen - 1) : uint64_t(f);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kRedThreads) void k_collapse(cv<T>* __restrict__ psi, uint64_t A, uint64_t mask,
+                                                          uint64_t val, T scale) {
+    const uint64_t stride = uint64_t(gridDim.x) * kRedThreads;
+    for (uint64_t s = uint64_t(blockIdx.x) * kRedThreads + threadIdx.x; s < A; s += stride) {
+        cv<T> v;
+        if ((s & mask) == val) {
+            v = psi[s];
+            v.x *= scale;
+            v.y *= scale;
+        } else {
+            v.x = 0;
+            v.y = 0;
+        }
+        psi[s] = v;
+    }
+}
+
+// Exact Ising energy sum_s |psi_s|^2 E(s), E(s) = -(sum h z + sum J z z).
+struct IsingArgs {
+    uint32_t n, ne;
+};
+
+template <typename T>
+__global__ __launch_bounds__(kRedThreads) void k_ising(const cv<T>* __restrict__ psi, uint64_t A, IsingArgs ia,
+                                                       const double* __restrict__ hv,
+                                                       const uint32_t* __restrict__ ei,
+                                                       const uint32_t* __restrict__ ej,
+                                                       const double* __restrict__ J,
+                                                       double* __restrict__ partial) {
+    __shared__ double lds[4];
+    __shared__ double sh[64], sJ[256];
+    __shared__ uint32_t si[256], sj[256];
+    for (uint32_t i = threadIdx.x; i < ia.n; i += kRedThreads) sh[i] = hv[i];
+    for (uint32_t k = threadIdx.x; k < ia.ne; k += kRedThreads) {
+        sJ[k] = J[k];
+        si[k] = ei[k];
+        sj[k] = ej[k];
+    }
+    __syncthreads();
+    double acc = 0;
+    const uint64_t stride = uint64_t(gridDim.x) * kRedThreads;
+    for (uint64_t s = uint64_t(blockIdx.x) * kRedThreads + threadIdx.x; s < A; s += stride) {
+        double e = 0;
+        for (uint32_t i = 0; i < ia.n; ++i) e += (s >> i & 1) ? -sh[i] : sh[i];
+        for (uint32_t k = 0; k < ia.ne; ++k) {
+            const uint64_t par = ((s >> si[k]) ^ (s >> sj[k])) & 1;
+            e += par ? -sJ[k] : sJ[k];
+        }
+        acc += norm2<T>(psi[s]) * (-e);
+    }
+    const double t = block_sum256(acc, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// ---------------------------------------------------------------------------
+// Host side.
+
+static int red_blocks(uint64_t work) {
+    uint64_t b = (work + kRedThreads * 4 - 1) / (kRedThreads * 4);
+    if (b > uint64_t(kRedBlocks)) b = kRedBlocks;
+    if (b < 1) b = 1;
+    return int(b);
+}
+
+static int fetch(qcs_register* r, const double* dev, int n, double* host) {
+    if (hipMemcpyAsync(r->hpin, dev, sizeof(double) * n, hipMemcpyDeviceToHost, r->stream) != hipSuccess)
+        return set_error(QCS_EDEVICE, "hipMemcpyAsync D2H failed");
+    if (hipStreamSynchronize(r->stream) != hipSuccess) return set_error(QCS_EDEVICE, "hipStreamSynchronize failed");
+    for (int i = 0; i < n; ++i) host[i] = r->hpin[i];
+    return QCS_OK;
+}
+
+template <typename T>
+static int rsum_sq_t(qcs_register* r, uint32_t first, uint32_t len, uint64_t val, double* out) {
+    const uint64_t norb = r->A >> len;
+    const int nb = red_blocks(norb);
+    hipLaunchKernelGGL((k_sum_sq<T>), dim3(nb), dim3(kRedThreads), 0, r->stream,
+                       static_cast<const cv<T>*>(r->psi), norb, first, len, val, r->red);
+    int e = check_launch(r, "k_sum_sq");
+    if (e) return e;
+    double* res = r->red + kRedBlocks * 3;
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(kRedThreads), 0, r->stream, r->red, nb, 1, res);
+    e = check_launch(r, "k_final_sum");
+    if (e) return e;
+    return fetch(r, res, 1, out);
+}
+
+int rsum_sq(qcs_register* r, uint32_t first, uint32_t len, uint64_t val, double* out) {
+    return r->prec == QCS_COMPLEX128 ? rsum_sq_t<double>(r, first, len, val, out)
+                                     : rsum_sq_t<float>(r, first, len, val, out);
+}
+
+template <typename Ta, typename Tb>
+static int rinner_t(qcs_register* a, qcs_register* b, double* re_im, double* max_abs) {
+    const int nb = red_blocks(a->A);
+    hipLaunchKernelGGL((k_inner<Ta, Tb>), dim3(nb), dim3(kRedThreads), 0, a->stream,
+                       static_cast<const cv<Ta>*>(a->psi), static_cast<const cv<Tb>*>(b->psi), a->A, a->red);
+    int e = check_launch(a, "k_inner");
+    if (e) return e;
+    double* res = a->red + kRedBlocks * 3;
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(kRedThreads), 0, a->stream, a->red, nb, 3, res);
+    hipLaunchKernelGGL(k_final_max, dim3(1), dim3(64), 0, a->stream, a->red, nb, res);
+    e = check_launch(a, "k_final_sum");
+    if (e) return e;
+    double tmp[3];
+    e = fetch(a, res, 3, tmp);
+    if (e) return e;
+    re_im[0] = tmp[0];
+    re_im[1] = tmp[1];
+    if (max_abs) *max_abs = tmp[2];
+    return QCS_OK;
+}
+
+int rinner(qcs_register* a, qcs_register* b, double* re_im, double* max_abs) {
+    // b's stream must be drained before a's stream reads it.
+    if (hipStreamSynchronize(b->stream) != hipSuccess) return set_error(QCS_EDEVICE, "sync failed");
+    if (a->prec == QCS_COMPLEX128) {
+        return b->prec == QCS_COMPLEX128 ? rinner_t<double, double>(a, b, re_im, max_abs)
+                                         : rinner_t<double, float>(a, b, re_im, max_abs);
+    }
+    return b->prec == QCS_COMPLEX128 ? rinner_t<float, double>(a, b, re_im, max_abs)
+                                     : rinner_t<float, float>(a, b, re_im, max_abs);
+}
+
+static ChunkGeom chunk_geom(const qcs_register* r) {
+    ChunkGeom g;
+    g.clen = r->A < (1ull << kChunkLog) ? r->A : (1ull << kChunkLog);
+    g.per = uint32_t((g.clen + kScanThreads - 1) / kScanThreads);
+    return g;
+}
+
+// Device scratch for per-shot inputs/outputs (grown on demand).
+struct ShotBuf {
+    double* probs = nullptr;
+    uint64_t* outs = nullptr;
+    uint64_t cap = 0;
+};
+
+template <typename T>
+static int cdf_search_t(qcs_register* r, uint64_t nshots, const double* probs_host, int strict,
+                        uint64_t dflt, uint64_t* out_host) {
+    const ChunkGeom g = chunk_geom(r);
+    const uint64_t nchunks = r->A / g.clen;
+    const auto* psi = static_cast<const cv<T>*>(r->psi);
+    hipLaunchKernelGGL((k_chunk_sums<T>), dim3(unsigned(nchunks)), dim3(kScanThreads), 0, r->stream, psi, g,
+                       r->chunk);
+    int e = check_launch(r, "k_chunk_sums");
+    if (e) return e;
+    hipLaunchKernelGGL(k_scan_chunks, dim3(1), dim3(1024), 0, r->stream, r->chunk, nchunks);
+    e = check_launch(r, "k_scan_chunks");
+    if (e) return e;
+    // per-shot buffers
+    double* dprobs = nullptr;
+    uint64_t* douts = nullptr;
+    if (hipMallocAsync(reinterpret_cast<void**>(&dprobs), sizeof(double) * nshots, r->stream) != hipSuccess ||
+        hipMallocAsync(reinterpret_cast<void**>(&douts), sizeof(uint64_t) * nshots, r->stream) != hipSuccess)
+        return set_error(QCS_ENOMEM, "shot buffers");
+    hipMemcpyAsync(dprobs, probs_host, sizeof(double) * nshots, hipMemcpyHostToDevice, r->stream);
+    for (uint64_t s0 = 0; s0 < nshots; s0 += 65535) {
+        const uint64_t cnt = nshots - s0 < 65535 ? nshots - s0 : 65535;
+        hipLaunchKernelGGL((k_search<T>), dim3(unsigned(cnt)), dim3(kScanThreads), 0, r->stream, psi, g, r->chunk,
+                           nchunks, dprobs + s0, strict, dflt, douts + s0);
+        e = check_launch(r, "k_search");
+        if (e) return e;
+    }
+    hipMemcpyAsync(out_host, douts, sizeof(uint64_t) * nshots, hipMemcpyDeviceToHost, r->stream);
+    hipFreeAsync(dprobs, r->stream);
+    hipFreeAsync(douts, r->stream);
+    if (hipStreamSynchronize(r->stream) != hipSuccess) return set_error(QCS_EDEVICE, "cdf search sync failed");
+    return QCS_OK;
+}
+
+int cdf_search(qcs_register* r, uint64_t nshots, const double* probs_host, int strict, uint64_t dflt,
+               uint64_t* out_host, int /*last_in_chunk_fallback*/) {
+    if (nshots == 0) return QCS_OK;
+    return r->prec == QCS_COMPLEX128 ? cdf_search_t<double>(r, nshots, probs_host, strict, dflt, out_host)
+                                     : cdf_search_t<float>(r, nshots, probs_host, strict, dflt, out_host);
+}
+
+int kcollapse(qcs_register* r, uint64_t mask, uint64_t val, double scale) {
+    const int nb = 4096;
+    if (r->prec == QCS_COMPLEX128)
+        hipLaunchKernelGGL((k_collapse<double>), dim3(nb), dim3(kRedThreads), 0, r->stream,
+                           static_cast<double2*>(r->psi), r->A, mask, val, scale);
+    else
+        hipLaunchKernelGGL((k_collapse<float>), dim3(nb), dim3(kRedThreads), 0, r->stream,
+                           static_cast<float2*>(r->psi), r->A, mask, val, float(scale));
+    return check_launch(r, "k_collapse");
+}
+
+template <typename T>
+static int rising_t(qcs_register* r, uint32_t ne, const uint32_t* ei, const uint32_t* ej, const double* J,
+                    const double* h, double* energy) {
+    if (r->n > 64 || ne > 256) return set_error(QCS_EINVAL, "ising: at most 64 sites and 256 edges");
+    // pack inputs into one device block
+    const size_t bytes = sizeof(double) * (r->n + ne) + sizeof(uint32_t) * 2 * ne + 64;
+    char* d = nullptr;
+    if (hipMallocAsync(reinterpret_cast<void**>(&d), bytes, r->stream) != hipSuccess)
+        return set_error(QCS_ENOMEM, "ising args");
+    std::vector<char> hbuf(bytes);
+    double* hh = reinterpret_cast<double*>(hbuf.data());
+    for (uint32_t i = 0; i < r->n; ++i) hh[i] = h[i];
+    for (uint32_t k = 0; k < ne; ++k) hh[r->n + k] = J[k];
+    uint32_t* hi = reinterpret_cast<uint32_t*>(hbuf.data() + sizeof(double) * (r->n + ne));
+    for (uint32_t k = 0; k < ne; ++k) {
+        hi[k] = ei[k];
+        hi[ne + k] = ej[k];
+    }
+    hipMemcpyAsync(d, hbuf.data(), bytes, hipMemcpyHostToDevice, r->stream);
+    const double* dh = reinterpret_cast<const double*>(d);
+    const uint32_t* dij = reinterpret_cast<const uint32_t*>(d + sizeof(double) * (r->n + ne));
+    const int nb = red_blocks(r->A);
+    IsingArgs ia{r->n, ne};
+    hipLaunchKernelGGL((k_ising<T>), dim3(nb), dim3(kRedThreads), 0, r->stream, static_cast<const cv<T>*>(r->psi),
+                       r->A, ia, dh, dij, dij + ne, dh + r->n, r->red);
+    int e = check_launch(r, "k_ising");
+    if (e) return e;
+    double* res = r->red + kRedBlocks * 3;
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(kRedThreads), 0, r->stream, r->red, nb, 1, res);
+    e = check_launch(r, "k_final_sum");
+    hipFreeAsync(d, r->stream);
+    if (e) return e;
+    return fetch(r, res, 1, energy);
+}
+
+int rising(qcs_register* r, uint32_t ne, const uint32_t* ei, const uint32_t* ej, const double* J, const double* h,
+           double* energy) {
+    return r->prec == QCS_COMPLEX128 ? rising_t<double>(r, ne, ei, ej, J, h, energy)
+                                     : rising_t<float>(r, ne, ei, ej, J, h, energy);
+}
+
+}  // namespace qcs

@@ -1,0 +1,65 @@
+"""Independent numpy references used to pin the oracle and the engine.
+
+- ``embed``: the dense Kronecker embedding of a k-qubit matrix in an n-qubit
+  register, the definition the reference gives in SimpleGates.h:211-232
+  (1 qubit), :311-333 (2 qubits), :411-434 (3 qubits): element (i, j) is
+  M[loc(i), loc(j)] when i and j agree outside the gate's qubits, else 0,
+  with loc() reading bit `qubit` as local bit 0, `controllingQubit1` as
+  local bit 1 and `controllingQubit2` as local bit 2.
+- QFT closed forms: QFT = sqrt(N) * ifft, IQFT = fft / sqrt(N) along the
+  transformed qubit range.
+"""
+import numpy as np
+
+
+def embed(m: np.ndarray, n: int, qubits) -> np.ndarray:
+    m = np.asarray(m, dtype=np.complex128)
+    N = 1 << n
+    mask = 0
+    for q in qubits:
+        mask |= 1 << q
+    idx = np.arange(N)
+    loc = np.zeros(N, dtype=np.int64)
+    for b, q in enumerate(qubits):
+        loc |= ((idx >> q) & 1) << b
+    same = (idx[:, None] | mask) == (idx[None, :] | mask)
+    return np.where(same, m[loc[:, None], loc[None, :]], 0)
+
+
+def apply_dense(psi, m, n, qubits):
+    return embed(m, n, qubits) @ psi
+
+
+def random_state(rng, n):
+    v = rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)
+    return v / np.linalg.norm(v)
+
+
+def random_unitary(rng, d):
+    z = (rng.standard_normal((d, d)) + 1j * rng.standard_normal((d, d))) / np.sqrt(2)
+    q, r = np.linalg.qr(z)
+    return q * (np.diagonal(r) / np.abs(np.diagonal(r)))
+
+
+def qft_closed(psi, n, sq=0, eq=None, inverse=False):
+    """QFT on qubits [sq, eq] of an n-qubit state via numpy.fft."""
+    eq = n - 1 if eq is None else eq
+    m = eq - sq + 1
+    t = np.asarray(psi).reshape(1 << (n - eq - 1), 1 << m, 1 << sq)
+    if inverse:
+        t = np.fft.fft(t, axis=1) / np.sqrt(1 << m)
+    else:
+        t = np.fft.ifft(t, axis=1) * np.sqrt(1 << m)
+    return t.reshape(-1)
+
+
+def cdf_first(psi, prob, dflt):
+    """First index with prob <= sequential cumsum(|psi|^2), else dflt.
+
+    |psi|^2 is re*re + im*im (libstdc++'s std::norm for double) and numpy's
+    cumsum accumulates sequentially, so this is the reference's scan
+    (QubitRegister.h:175-190) to the last bit."""
+    psi = np.asarray(psi)
+    c = np.cumsum(psi.real * psi.real + psi.imag * psi.imag)
+    i = int(np.searchsorted(c, prob, side="left"))
+    return i if i < c.shape[0] else dflt

@@ -1,0 +1,299 @@
+"""GPU parity: the HIP engine through the C-ABI against the CPU oracle.
+
+Bars: FP64 amplitudes within 1e-12 max-abs of the oracle; complex64 within
+2e-6 (single-precision storage, FP32 arithmetic); measurement outcomes equal
+to the oracle's for the same supplied uniforms unless the uniform lies within
+the stated rounding margin of a CDF boundary.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from gatecatalog import catalog, placements
+from oracle.oracle import OracleRegister
+from qcsim_amd import QuantumFourierTransform, QubitRegister
+from qcsim_amd import gates as G
+from refmath import cdf_first, qft_closed, random_state
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+TOL64 = 1e-12
+TOL32 = 2e-6
+
+
+def pair(n, psi, precision="c128"):
+    g = QubitRegister(n, precision=precision)
+    g.setRegisterStorageFastNoNormalize(psi)
+    o = OracleRegister(n)
+    o.set_state(psi)
+    return g, o
+
+
+def cdf_margin(psi, probs):
+    """Distance from each prob to the nearest CDF value (sequential cumsum)."""
+    p = psi.real * psi.real + psi.imag * psi.imag
+    c = np.cumsum(p)
+    i = np.clip(np.searchsorted(c, probs), 0, c.shape[0] - 1)
+    lo = np.abs(c[i] - probs)
+    lo2 = np.abs(c[np.clip(i - 1, 0, None)] - probs)
+    return np.minimum(lo, lo2)
+
+
+@pytest.mark.parametrize("precision,tol", [("c128", TOL64), ("c64", TOL32)])
+@pytest.mark.parametrize("n", [7, 13])
+def test_every_gate_family(precision, tol, n):
+    rng = np.random.default_rng(100 + n)
+    for group, fams in enumerate(catalog(n)):
+        k = group + 1
+        for name, gate in fams:
+            for qs in placements(n, k, rng, 4):
+                psi = random_state(rng, n)
+                g, o = pair(n, psi, precision)
+                g.ApplyGate(gate, *qs)
+                o.apply_gate(gate, *qs)
+                err = np.max(np.abs(g.getRegisterStorage() - o.state()))
+                assert err < tol, (precision, name, qs, err)
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5])
+def test_dense_blocks(k):
+    from refmath import random_unitary
+
+    n = 11
+    rng = np.random.default_rng(k)
+    for _ in range(4):
+        qs = [int(x) for x in rng.choice(n, size=k, replace=False)]
+        m = random_unitary(rng, 1 << k)
+        psi = random_state(rng, n)
+        g, o = pair(n, psi)
+        g.apply_dense(qs, m)
+        o.apply_dense(qs, m)
+        assert np.max(np.abs(g.getRegisterStorage() - o.state())) < TOL64, qs
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 9, 12, 14])
+def test_qft_matches_oracle_and_closed_form(n):
+    rng = np.random.default_rng(n)
+    psi = random_state(rng, n)
+    for inverse in (False, True):
+        g, o = pair(n, psi)
+        g.qft(inverse=inverse)
+        o.qft(inverse=inverse)
+        got = g.getRegisterStorage()
+        assert np.max(np.abs(got - o.state())) < TOL64
+        assert np.max(np.abs(got - qft_closed(psi, n, inverse=inverse))) < TOL64
+
+
+def test_qft_subranges_and_facade():
+    n = 10
+    rng = np.random.default_rng(77)
+    psi = random_state(rng, n)
+    for sq, eq, swap in [(0, 4, True), (3, 8, True), (2, 9, False), (6, 6, True), (1, 1000, True)]:
+        g, o = pair(n, psi)
+        qft = QuantumFourierTransform(n, sq, eq)
+        qft.QFT(g, swap)
+        o.qft(sq, eq, swap)
+        assert np.max(np.abs(g.getRegisterStorage() - o.state())) < TOL64, (sq, eq)
+        qft.IQFT(g, swap)
+        assert np.max(np.abs(g.getRegisterStorage() - psi)) < TOL64
+
+
+def test_measurement_known_answers():
+    with open(os.path.join(GOLDEN, "measurement_known_answers.json")) as f:
+        cases = json.load(f)
+    psi = np.array([0.5, -0.5j, 0, 1 / math.sqrt(2)])
+    for c in cases:
+        g = QubitRegister(2)
+        g.setRawAmplitude(0, 0.5)
+        g.setRawAmplitude(1, -0.5j)
+        g.setRawAmplitude(3, 1 / math.sqrt(2))
+        g.Normalize()
+        assert np.max(np.abs(g.getRegisterStorage() - psi)) < 1e-15
+        assert g.MeasureQubit(c["qubit"], u=c["u"]) == c["outcome"]
+        want = np.array([complex(*z) for z in c["amplitudes"]])
+        assert np.max(np.abs(g.getRegisterStorage() - want)) < 1e-12, c
+
+
+@pytest.mark.parametrize("n", [3, 11, 16])
+def test_cdf_searches(n):
+    rng = np.random.default_rng(n + 1)
+    psi = random_state(rng, n)
+    psi[rng.choice(1 << n, size=(1 << n) // 4, replace=False)] = 0
+    psi /= np.linalg.norm(psi)
+    g, o = pair(n, psi)
+    us = np.concatenate([rng.random(300), [0.0, 0.5, 1 - 2**-53]])
+    probs = 1.0 - us
+    margin = cdf_margin(psi, probs)
+    mism = 0
+    for u, m in zip(us, margin):
+        a = g.MeasureNoCollapse(u=u)
+        b = o.measure_no_collapse(u)
+        if a != b:
+            mism += 1
+            assert m < 1e-13 * (1 << n), (u, a, b, m)
+    assert mism <= 1
+    # range, single qubit and sampling
+    for first, second in [(0, 0), (1, n - 1), (n - 1, n - 1)]:
+        for u in us[:40]:
+            assert g.MeasureNoCollapse(first, second, u=u) == o.measure_no_collapse(u, first, second)
+    s_g = g.sample(us)
+    s_o = o.sample(us)
+    bad = np.nonzero(s_g != s_o)[0]
+    assert all(margin[i] < 1e-13 * (1 << n) for i in bad)
+    s_g = g.sample(us, 1, min(3, n - 1))
+    s_o = o.sample(us, 1, min(3, n - 1))
+    assert np.count_nonzero(s_g != s_o) <= len(bad)
+
+
+def test_measure_defaults_on_unnormalised_state():
+    n = 6
+    rng = np.random.default_rng(1)
+    psi = 0.5 * random_state(rng, n)
+    g, o = pair(n, psi)
+    assert g.MeasureNoCollapse(u=0.0) == o.measure_no_collapse(0.0) == 0
+    assert g.MeasureAll(u=0.0) == o.measure_all(0.0) == (1 << n) - 1
+    assert np.max(np.abs(g.getRegisterStorage() - o.state())) == 0
+
+
+@pytest.mark.parametrize("n", [4, 12])
+def test_measure_collapse(n):
+    rng = np.random.default_rng(3 * n)
+    psi = random_state(rng, n)
+    for first, second in [(0, 0), (n - 1, n - 1), (1, 2), (0, n - 1)]:
+        for u in (0.13, 0.5, 0.91):
+            g, o = pair(n, psi)
+            a = g.Measure(first, second, u=u)
+            b = o.measure(first, second, u)
+            assert a == b
+            assert np.max(np.abs(g.getRegisterStorage() - o.state())) < TOL64
+    for u in (0.2, 0.7):
+        g, o = pair(n, psi)
+        assert g.MeasureAll(u=u) == o.measure_all(u)
+        assert np.max(np.abs(g.getRegisterStorage() - o.state())) == 0
+
+
+def test_reductions_and_state_utilities():
+    n = 12
+    rng = np.random.default_rng(8)
+    psi = random_state(rng, n)
+    g, o = pair(n, psi)
+    for q in (0, 5, n - 1):
+        assert abs(g.GetQubitProbability(q) - o.qubit_probability(q)) < 1e-14
+    assert abs(g.norm2() - 1) < 1e-13
+    g.setRegisterStorageFastNoNormalize(psi * 3)
+    g.Normalize()
+    assert np.max(np.abs(g.getRegisterStorage() - psi)) < 1e-15
+    h = QubitRegister(n)
+    h.setRegisterStorageFastNoNormalize(psi)
+    assert abs(h.inner(g) - 1) < 1e-13
+    assert abs(g.stateFidelity(psi) - 1) < 1e-13
+    g.setToBasisState(5)
+    assert g.getBasisStateAmplitude(5) == 1 and g.getBasisStateAmplitude(4) == 0
+    assert g.getBasisStateAmplitude(1 << n) == 0  # out of range: 0, silently
+    g.setToBasisState(1 << n)  # out of range: no-op
+    assert g.getBasisStateAmplitude(5) == 1
+    g.setToCatState()
+    s = 1 / math.sqrt(2)
+    assert abs(g.getBasisStateAmplitude(0) - s) < 1e-16 and abs(g.getBasisStateAmplitude((1 << n) - 1) - s) < 1e-16
+    g.setToEqualSuperposition()
+    assert abs(g.norm2() - 1) < 1e-12
+    g.setRegisterStorageFastNoNormalize(psi * (0.3 - 0.4j))
+    g.AdjustPhaseAndNormalize()
+    v = g.getRegisterStorage()
+    assert abs(v[0].imag) < 1e-15 and v[0].real > 0 and abs(np.linalg.norm(v) - 1) < 1e-13
+
+
+def test_expectation_values_and_record_replay():
+    # Tests.cpp:665-723
+    z0 = G.AppliedGate(G.PauliZGate().getRawOperatorMatrix(), 0)
+    r = QubitRegister(3)
+    assert r.ExpectationValue([]) == 1.0
+    assert abs(r.ExpectationValue([z0]) - 1) < 1e-12
+    r.ApplyGate(G.PauliXGate(), 0)
+    assert abs(r.ExpectationValue([z0]) + 1) < 1e-12
+    r.ApplyGate(G.HadamardGate(), 0)
+    assert abs(r.ExpectationValue([z0])) < 1e-12
+    r.setToBasisState(0)
+    r.ApplyGate(G.HadamardGate(), 1)
+    assert abs(r.ExpectationValue([G.AppliedGate(G.PauliZGate().getRawOperatorMatrix(), 1)])) < 1e-12
+    # Compute / Uncompute restores the state
+    n = 8
+    rng = np.random.default_rng(4)
+    psi = random_state(rng, n)
+    r = QubitRegister(n)
+    r.setRegisterStorageFastNoNormalize(psi)
+    r.ComputeStart()
+    r.ApplyGate(G.ToffoliGate(), 0, 3, 5)
+    r.ApplyGate(G.SingleQubitGate(G.haar_u2(rng)), 2)
+    r.ApplyGate(G.ControlledRyGate(0.4), 6, 1)
+    r.ComputeEnd()
+    r.Uncompute()
+    assert np.max(np.abs(r.getRegisterStorage() - psi)) < 1e-13
+    r.SaveState()
+    r.ApplyGate(G.HadamardGate(), 0)
+    r.RestoreState()
+    assert np.max(np.abs(r.getRegisterStorage() - psi)) < 1e-13
+
+
+@pytest.mark.parametrize(
+    "gate,args,msg",
+    [
+        (G.HadamardGate(), (9,), "Qubit number is too high"),
+        (G.CNOTGate(), (0, 9), "Controlling qubit number is too high"),
+        (G.CNOTGate(), (2, 2), "Qubit and controlling qubit are the same"),
+        (G.ToffoliGate(), (0, 1, 1), "Qubits must be different"),
+    ],
+)
+def test_invalid_arguments(gate, args, msg):
+    r = QubitRegister(4)
+    with pytest.raises(ValueError, match=msg):
+        r.ApplyGate(gate, *args)
+
+
+def test_ising_energy_matches_oracle():
+    n = 10
+    rng = np.random.default_rng(12)
+    edges = [(i, (i + 1) % n) for i in range(n)] + [(i, (i + 3) % n) for i in range(0, n, 2)]
+    edges = [(min(a, b), max(a, b)) for a, b in edges]
+    J = rng.standard_normal(len(edges))
+    h = 0.5 * rng.standard_normal(n)
+    psi = random_state(rng, n)
+    g, o = pair(n, psi)
+    assert abs(g.ising_energy(edges, J, h) - o.ising_energy(edges, J, h)) < 1e-12
+
+
+def test_random_circuit_config0():
+    """20 qubits complex128: 40 layers of Haar U(2) on every qubit + CNOT/CZ
+    on a random pairing, then a 20-qubit QFT; 1024 supplied uniforms."""
+    n, layers = 20, 40
+    rng = np.random.default_rng(20240)
+    g = QubitRegister(n)
+    o = OracleRegister(n)
+    for _ in range(layers):
+        for q in range(n):
+            u = G.SingleQubitGate(G.haar_u2(rng))
+            g.ApplyGate(u, q)
+            o.apply_gate(u, q)
+        perm = rng.permutation(n)
+        for a, b in zip(perm[0::2], perm[1::2]):
+            gate = G.CNOTGate() if rng.random() < 0.5 else G.ControlledZGate()
+            g.ApplyGate(gate, int(a), int(b))
+            o.apply_gate(gate, int(a), int(b))
+    g.qft()
+    o.qft()
+    ref = o.state()
+    got = g.getRegisterStorage()
+    err = float(np.max(np.abs(got - ref)))
+    infid = 1 - abs(np.vdot(ref, got)) ** 2
+    print(f"\nconfig0: max-abs {err:.3e} infidelity {infid:.3e}")
+    assert err < TOL64
+    us = rng.random(1024)
+    margin = cdf_margin(ref, 1 - us)
+    sg = g.sample(us)
+    so = o.sample(us)
+    bad = np.nonzero(sg != so)[0]
+    print(f"config0: {len(bad)} sample mismatches, min CDF margin {margin.min():.3e}")
+    assert all(margin[i] < 1e-10 for i in bad)
