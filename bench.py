@@ -29,13 +29,18 @@ def algorithmic_bytes_1q(n, amp_bytes=16):
     return 2 * amp_bytes * (1 << n)
 
 
-def gpu_sweep(n, steps, warmup, seed=7):
+def gpu_sweep(n, steps, warmup, seed=7, tune=True):
     from qcsim_amd import QubitRegister
     from qcsim_amd import gates as G
 
     rng = np.random.default_rng(seed)
     reg = QubitRegister(n)
     reg.setToEqualSuperposition()
+    tuned = None
+    if tune:  # launch variant per target: from the autotuner cache, else measured now (untimed setup)
+        from qcsim_amd.tuning import ensure_tuned
+
+        tuned = ensure_tuned(reg)
     mats = [G.SingleQubitGate(G.haar_u2(rng)) for _ in range(n)]
 
     def step():
@@ -54,7 +59,7 @@ def gpu_sweep(n, steps, warmup, seed=7):
     wall = time.perf_counter() - t0
     norm_err = abs(reg.norm2() - 1.0)
     reg.close()
-    return ev_ms, wall, norm_err
+    return ev_ms, wall, norm_err, tuned
 
 
 def cpu_baseline(n, targets, seed=7):
@@ -88,6 +93,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-tune", action="store_true", help="use the default launch variants")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -97,7 +103,7 @@ def main():
         )
         sys.exit(2)
     n = args.qubits
-    ev_ms, wall, norm_err = gpu_sweep(n, args.steps, args.warmup)
+    ev_ms, wall, norm_err, tuned = gpu_sweep(n, args.steps, args.warmup, tune=not args.no_tune)
     launches = n * args.steps
     gates_per_s = launches / (ev_ms / 1e3)
     avg_launch_s = ev_ms / 1e3 / launches
@@ -123,7 +129,7 @@ def main():
             "workload": f"{n}-qubit complex128 statevector, Haar U(2) sweep over targets 0..{n - 1} (one step = {n} gates)",
             "qubits": n,
             "state_bytes": (1 << n) * 16,
-            "kernel": "k_1q<double,0,4> (generic 1-qubit, in place)",
+            "kernel": "k_1q<double,0,P,NT,XCD> (generic 1-qubit, in place; launch variant per target from the autotuner)",
         },
         "roofline": {
             "bound": "hbm",
@@ -134,6 +140,7 @@ def main():
             "traffic": None,
         },
         "cpu_baseline": cpu,
+        "launch_variants": tuned or "default",
         "host_wall_s": round(wall, 4),
         "norm_drift": norm_err,
     }

@@ -170,6 +170,20 @@ int qcs_ising_energy(qcs_register* r, uint32_t n_edges, const uint32_t* ei,
                      const uint32_t* ej, const double* J, const double* h,
                      double* energy);
 
+/* ---- autotuning (no counterpart in the reference) ------------------------ */
+
+/* Times every launch variant of the streaming gate kernels on this register
+ * (identity gates: the state keeps its values) for each qubit, and keeps the
+ * fastest per (precision, qubit count, qubit) for all later launches in the
+ * process. gbps, if not NULL, receives nqubits * qcs_autotune_variants()
+ * algorithmic GB/s figures. */
+int qcs_autotune(qcs_register* r, uint32_t reps, double* gbps);
+int qcs_autotune_variants(void);
+/* Read / preset one entry of that table (vid = -1: untuned default), so a
+ * tuning result saved by one process can be reused by the next. */
+int qcs_set_tuned_variant(int precision, uint32_t n, uint32_t q, int vid);
+int qcs_get_tuned_variant(int precision, uint32_t n, uint32_t q);
+
 /* ---- stream / timing --------------------------------------------------- */
 int qcs_synchronize(qcs_register* r);
 /* HIP events on the register's stream around the enclosed calls. */

@@ -64,6 +64,10 @@ SIGNATURES = {
     "qcs_inner": (_I, [_P, _P, _PD]),
     "qcs_compare": (_I, [_P, _P, _PD, _PD]),
     "qcs_ising_energy": (_I, [_P, _U32, _PU32, _PU32, _PD, _PD, _PD]),
+    "qcs_autotune": (_I, [_P, _U32, _PD]),
+    "qcs_autotune_variants": (_I, []),
+    "qcs_set_tuned_variant": (_I, [_I, _U32, _U32, _I]),
+    "qcs_get_tuned_variant": (_I, [_I, _U32, _U32]),
     "qcs_synchronize": (_I, [_P]),
     "qcs_timer_begin": (_I, [_P]),
     "qcs_timer_end": (_I, [_P, C.POINTER(C.c_float)]),

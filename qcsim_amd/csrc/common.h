@@ -51,6 +51,45 @@ __device__ __forceinline__ uint64_t insert_run(uint64_t o, uint32_t pos, uint32_
     return ((o & ~low) << len) | (o & low);
 }
 
+// Native clang vectors for the nontemporal builtins (HIP's double2/float2
+// are structs, which the builtins do not take).
+template <typename T> struct nvec;
+template <> struct nvec<double> { typedef double type __attribute__((ext_vector_type(2))); };
+template <> struct nvec<float> { typedef float type __attribute__((ext_vector_type(2))); };
+
+// Load / store one amplitude; NT = nontemporal (streaming, no reuse).
+template <typename T, bool NT>
+__device__ __forceinline__ cv<T> ldamp(const cv<T>* p) {
+    if (NT) {
+        const auto v = __builtin_nontemporal_load(reinterpret_cast<const typename nvec<T>::type*>(p));
+        cv<T> r;
+        r.x = v.x;
+        r.y = v.y;
+        return r;
+    }
+    return *p;
+}
+
+template <typename T, bool NT>
+__device__ __forceinline__ void stamp(cv<T>* p, cv<T> a) {
+    if (NT) {
+        typename nvec<T>::type v;
+        v.x = a.x;
+        v.y = a.y;
+        __builtin_nontemporal_store(v, reinterpret_cast<typename nvec<T>::type*>(p));
+    } else {
+        *p = a;
+    }
+}
+
+// Block index remap so that the 8 XCDs (blocks are dispatched round-robin
+// over them) each stream one contiguous eighth of the state.
+template <bool XCD>
+__device__ __forceinline__ uint64_t block_id() {
+    if (XCD && (gridDim.x & 7) == 0) return uint64_t(blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+    return blockIdx.x;
+}
+
 template <typename T>
 __device__ __forceinline__ cv<T> cmul(T ar, T ai, cv<T> x) {
     cv<T> r;
@@ -90,7 +129,21 @@ constexpr int kScanThreads = 256;
 int set_error(int code, const char* fmt, ...);
 int check_launch(qcs_register* r, const char* what);
 
+// Launch variant of the streaming gate kernels: orbits per thread (1, 2, 4),
+// nontemporal access and XCD-aware block order. Chosen per (precision,
+// qubit count, lowest touched qubit) by the autotuner (autotune.hip), with a
+// measured default when untuned.
+struct Variant {
+    uint8_t p, nt, xcd;
+};
+Variant pick_variant(const qcs_register* r, uint32_t lowbit);
+int variant_count();
+Variant variant_at(int i);
+void set_tuned(int prec, uint32_t n, uint32_t q, int vid);
+
 // gate_kernels.hip
+int k1q_variant(qcs_register* r, int mode, uint32_t target, const uint32_t* ctrls, int nctrl,
+                const hcplx* m, Variant v);
 int k1q(qcs_register* r, int mode, uint32_t target, const uint32_t* ctrls, int nctrl,
         const hcplx* m /* 4 */);
 int kphase(qcs_register* r, const uint32_t* bits, int nbits, hcplx ph);

@@ -36,10 +36,10 @@ static unsigned grid_for(uint64_t norb, int per_block) {
 // 1-qubit update on the orbit {i0, i0|bit} (plus controls in cmask).
 // MODE 0: generic 2x2 (K3, K8, K13); 1: antidiagonal (K2, K7, K12);
 // 2: diagonal with both entries (K1, K6, K11).
-template <typename T, int MODE, int P>
+template <typename T, int MODE, int P, bool NT, bool XCD>
 __global__ __launch_bounds__(kThreads) void k_1q(cv<T>* __restrict__ psi, uint64_t norb, Ins ins,
                                                  uint64_t cmask, uint64_t bit, MatD<T, 2> m) {
-    const uint64_t base = uint64_t(blockIdx.x) * (kThreads * P) + threadIdx.x;
+    const uint64_t base = block_id<XCD>() * (kThreads * P) + threadIdx.x;
     cv<T> a[P], b[P];
     uint64_t i0[P];
 #pragma unroll
@@ -47,8 +47,8 @@ __global__ __launch_bounds__(kThreads) void k_1q(cv<T>* __restrict__ psi, uint64
         const uint64_t o = base + uint64_t(j) * kThreads;
         i0[j] = insert_zeros(o, ins) | cmask;
         if (o < norb) {
-            a[j] = psi[i0[j]];
-            b[j] = psi[i0[j] | bit];
+            a[j] = ldamp<T, NT>(psi + i0[j]);
+            b[j] = ldamp<T, NT>(psi + (i0[j] | bit));
         }
     }
 #pragma unroll
@@ -68,39 +68,39 @@ __global__ __launch_bounds__(kThreads) void k_1q(cv<T>* __restrict__ psi, uint64
             u = cmul<T>(m.re[0], m.im[0], a[j]);
             v = cmul<T>(m.re[3], m.im[3], b[j]);
         }
-        psi[i0[j]] = u;
-        psi[i0[j] | bit] = v;
+        stamp<T, NT>(psi + i0[j], u);
+        stamp<T, NT>(psi + (i0[j] | bit), v);
     }
 }
 
 // Phase on the single amplitude of each orbit whose listed bits are all 1:
 // P/S/T/Z (m00 == 1), CZ/CP/CRz-with-unit-m22 (K6), CCZ (K11).
-template <typename T, int P>
+template <typename T, int P, bool NT, bool XCD>
 __global__ __launch_bounds__(kThreads) void k_phase(cv<T>* __restrict__ psi, uint64_t norb, Ins ins,
                                                     uint64_t cmask, T pr, T pi) {
-    const uint64_t base = uint64_t(blockIdx.x) * (kThreads * P) + threadIdx.x;
+    const uint64_t base = block_id<XCD>() * (kThreads * P) + threadIdx.x;
     cv<T> a[P];
     uint64_t i0[P];
 #pragma unroll
     for (int j = 0; j < P; ++j) {
         const uint64_t o = base + uint64_t(j) * kThreads;
         i0[j] = insert_zeros(o, ins) | cmask;
-        if (o < norb) a[j] = psi[i0[j]];
+        if (o < norb) a[j] = ldamp<T, NT>(psi + i0[j]);
     }
 #pragma unroll
     for (int j = 0; j < P; ++j) {
         const uint64_t o = base + uint64_t(j) * kThreads;
-        if (o < norb) psi[i0[j]] = cmul<T>(pr, pi, a[j]);
+        if (o < norb) stamp<T, NT>(psi + i0[j], cmul<T>(pr, pi, a[j]));
     }
 }
 
 // Exchange psi[i|ba] <-> psi[i|bb], optionally times a phase (SWAP K4,
 // iSWAP/iSWAP-dagger K5, Fredkin K10 with its control in cmask).
-template <typename T, bool PHASED, int P>
+template <typename T, bool PHASED, int P, bool NT, bool XCD>
 __global__ __launch_bounds__(kThreads) void k_swap(cv<T>* __restrict__ psi, uint64_t norb, Ins ins,
                                                    uint64_t cmask, uint64_t ba, uint64_t bb, T pr,
                                                    T pi) {
-    const uint64_t base = uint64_t(blockIdx.x) * (kThreads * P) + threadIdx.x;
+    const uint64_t base = block_id<XCD>() * (kThreads * P) + threadIdx.x;
     cv<T> a[P], b[P];
     uint64_t i0[P];
 #pragma unroll
@@ -108,8 +108,8 @@ __global__ __launch_bounds__(kThreads) void k_swap(cv<T>* __restrict__ psi, uint
         const uint64_t o = base + uint64_t(j) * kThreads;
         i0[j] = insert_zeros(o, ins) | cmask;
         if (o < norb) {
-            a[j] = psi[i0[j] | ba];
-            b[j] = psi[i0[j] | bb];
+            a[j] = ldamp<T, NT>(psi + (i0[j] | ba));
+            b[j] = ldamp<T, NT>(psi + (i0[j] | bb));
         }
     }
 #pragma unroll
@@ -117,11 +117,11 @@ __global__ __launch_bounds__(kThreads) void k_swap(cv<T>* __restrict__ psi, uint
         const uint64_t o = base + uint64_t(j) * kThreads;
         if (o >= norb) continue;
         if (PHASED) {
-            psi[i0[j] | ba] = cmul<T>(pr, pi, b[j]);
-            psi[i0[j] | bb] = cmul<T>(pr, pi, a[j]);
+            stamp<T, NT>(psi + (i0[j] | ba), cmul<T>(pr, pi, b[j]));
+            stamp<T, NT>(psi + (i0[j] | bb), cmul<T>(pr, pi, a[j]));
         } else {
-            psi[i0[j] | ba] = b[j];
-            psi[i0[j] | bb] = a[j];
+            stamp<T, NT>(psi + (i0[j] | ba), b[j]);
+            stamp<T, NT>(psi + (i0[j] | bb), a[j]);
         }
     }
 }
@@ -229,15 +229,30 @@ static unsigned stream_grid(uint64_t A) {
     return unsigned(g);
 }
 
+// Variant dispatch: (P, NT, XCD) -> template instance.
+#define QCS_VARIANT_SWITCH(v, LAUNCH)                                    \
+    do {                                                                 \
+        if (v.p == 1 && !v.nt && !v.xcd) LAUNCH(1, false, false);        \
+        else if (v.p == 1 && v.nt && !v.xcd) LAUNCH(1, true, false);     \
+        else if (v.p == 1 && v.nt && v.xcd) LAUNCH(1, true, true);       \
+        else if (v.p == 2 && !v.nt && v.xcd) LAUNCH(2, false, true);     \
+        else if (v.p == 2 && v.nt && !v.xcd) LAUNCH(2, true, false);     \
+        else if (v.p == 4 && !v.nt && v.xcd) LAUNCH(4, false, true);     \
+        else if (v.p == 4 && v.nt && v.xcd) LAUNCH(4, true, true);       \
+        else LAUNCH(2, true, true);                                      \
+    } while (0)
+
 template <typename T>
 static int k1q_t(qcs_register* r, int mode, uint32_t target, const uint32_t* ctrls, int nctrl,
-                 const hcplx* m) {
+                 const hcplx* m, Variant v) {
     uint32_t pos[6];
     uint64_t cmask = 0;
+    uint32_t low = target;
     pos[0] = target;
     for (int i = 0; i < nctrl; ++i) {
         pos[1 + i] = ctrls[i];
         cmask |= 1ull << ctrls[i];
+        low = ctrls[i] < low ? ctrls[i] : low;
     }
     const Ins ins = make_ins(pos, 1 + nctrl);
     const uint64_t norb = r->A >> (1 + nctrl);
@@ -246,35 +261,55 @@ static int k1q_t(qcs_register* r, int mode, uint32_t target, const uint32_t* ctr
         mm.re[i] = T(m[i].re);
         mm.im[i] = T(m[i].im);
     }
-    constexpr int P = 4;
-    const unsigned g = grid_for(norb, kThreads * P);
+    if (v.p == 0) v = pick_variant(r, low);
     auto* psi = static_cast<cv<T>*>(r->psi);
     const uint64_t bit = 1ull << target;
-    if (mode == 0)
-        hipLaunchKernelGGL((k_1q<T, 0, P>), dim3(g), dim3(kThreads), 0, r->stream, psi, norb, ins, cmask, bit, mm);
-    else if (mode == 1)
-        hipLaunchKernelGGL((k_1q<T, 1, P>), dim3(g), dim3(kThreads), 0, r->stream, psi, norb, ins, cmask, bit, mm);
-    else
-        hipLaunchKernelGGL((k_1q<T, 2, P>), dim3(g), dim3(kThreads), 0, r->stream, psi, norb, ins, cmask, bit, mm);
+#define QCS_L1Q(PP, NTT, XX)                                                                                   \
+    do {                                                                                                       \
+        const unsigned g = grid_for(norb, kThreads * PP);                                                      \
+        if (mode == 0)                                                                                         \
+            hipLaunchKernelGGL((k_1q<T, 0, PP, NTT, XX>), dim3(g), dim3(kThreads), 0, r->stream, psi, norb, ins, \
+                               cmask, bit, mm);                                                                \
+        else if (mode == 1)                                                                                    \
+            hipLaunchKernelGGL((k_1q<T, 1, PP, NTT, XX>), dim3(g), dim3(kThreads), 0, r->stream, psi, norb, ins, \
+                               cmask, bit, mm);                                                                \
+        else                                                                                                   \
+            hipLaunchKernelGGL((k_1q<T, 2, PP, NTT, XX>), dim3(g), dim3(kThreads), 0, r->stream, psi, norb, ins, \
+                               cmask, bit, mm);                                                                \
+    } while (0)
+    QCS_VARIANT_SWITCH(v, QCS_L1Q);
+#undef QCS_L1Q
     return check_launch(r, "k_1q");
+}
+
+int k1q_variant(qcs_register* r, int mode, uint32_t target, const uint32_t* ctrls, int nctrl,
+                const hcplx* m, Variant v) {
+    return r->prec == QCS_COMPLEX128 ? k1q_t<double>(r, mode, target, ctrls, nctrl, m, v)
+                                     : k1q_t<float>(r, mode, target, ctrls, nctrl, m, v);
 }
 
 int k1q(qcs_register* r, int mode, uint32_t target, const uint32_t* ctrls, int nctrl,
         const hcplx* m) {
-    return r->prec == QCS_COMPLEX128 ? k1q_t<double>(r, mode, target, ctrls, nctrl, m)
-                                     : k1q_t<float>(r, mode, target, ctrls, nctrl, m);
+    return k1q_variant(r, mode, target, ctrls, nctrl, m, Variant{0, 0, 0});
 }
 
 template <typename T>
 static int kphase_t(qcs_register* r, const uint32_t* bits, int nbits, hcplx ph) {
     uint64_t cmask = 0;
-    for (int i = 0; i < nbits; ++i) cmask |= 1ull << bits[i];
+    uint32_t low = 63;
+    for (int i = 0; i < nbits; ++i) {
+        cmask |= 1ull << bits[i];
+        low = bits[i] < low ? bits[i] : low;
+    }
     const Ins ins = make_ins(bits, nbits);
     const uint64_t norb = r->A >> nbits;
-    constexpr int P = 4;
-    const unsigned g = grid_for(norb, kThreads * P);
-    hipLaunchKernelGGL((k_phase<T, P>), dim3(g), dim3(kThreads), 0, r->stream,
-                       static_cast<cv<T>*>(r->psi), norb, ins, cmask, T(ph.re), T(ph.im));
+    const Variant v = pick_variant(r, low);
+    auto* psi = static_cast<cv<T>*>(r->psi);
+#define QCS_LPH(PP, NTT, XX)                                                                                   \
+    hipLaunchKernelGGL((k_phase<T, PP, NTT, XX>), dim3(grid_for(norb, kThreads * PP)), dim3(kThreads), 0,      \
+                       r->stream, psi, norb, ins, cmask, T(ph.re), T(ph.im))
+    QCS_VARIANT_SWITCH(v, QCS_LPH);
+#undef QCS_LPH
     return check_launch(r, "k_phase");
 }
 
@@ -288,21 +323,28 @@ static int kswap_t(qcs_register* r, uint32_t a, uint32_t b, const uint32_t* ctrl
                    hcplx ph, bool phased) {
     uint32_t pos[6] = {a, b};
     uint64_t cmask = 0;
+    uint32_t low = a < b ? a : b;
     for (int i = 0; i < nctrl; ++i) {
         pos[2 + i] = ctrls[i];
         cmask |= 1ull << ctrls[i];
+        low = ctrls[i] < low ? ctrls[i] : low;
     }
     const Ins ins = make_ins(pos, 2 + nctrl);
     const uint64_t norb = r->A >> (2 + nctrl);
-    constexpr int P = 2;
-    const unsigned g = grid_for(norb, kThreads * P);
+    const Variant v = pick_variant(r, low);
     auto* psi = static_cast<cv<T>*>(r->psi);
-    if (phased)
-        hipLaunchKernelGGL((k_swap<T, true, P>), dim3(g), dim3(kThreads), 0, r->stream, psi, norb, ins,
-                           cmask, 1ull << a, 1ull << b, T(ph.re), T(ph.im));
-    else
-        hipLaunchKernelGGL((k_swap<T, false, P>), dim3(g), dim3(kThreads), 0, r->stream, psi, norb, ins,
-                           cmask, 1ull << a, 1ull << b, T(0), T(0));
+#define QCS_LSW(PP, NTT, XX)                                                                                   \
+    do {                                                                                                       \
+        const unsigned g = grid_for(norb, kThreads * PP);                                                      \
+        if (phased)                                                                                            \
+            hipLaunchKernelGGL((k_swap<T, true, PP, NTT, XX>), dim3(g), dim3(kThreads), 0, r->stream, psi, norb, \
+                               ins, cmask, 1ull << a, 1ull << b, T(ph.re), T(ph.im));                          \
+        else                                                                                                   \
+            hipLaunchKernelGGL((k_swap<T, false, PP, NTT, XX>), dim3(g), dim3(kThreads), 0, r->stream, psi,    \
+                               norb, ins, cmask, 1ull << a, 1ull << b, T(0), T(0));                            \
+    } while (0)
+    QCS_VARIANT_SWITCH(v, QCS_LSW);
+#undef QCS_LSW
     return check_launch(r, "k_swap");
 }
 

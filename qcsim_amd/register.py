@@ -341,6 +341,14 @@ class QubitRegister:
         qr._record = self._record
         return qr
 
+    def autotune(self, reps: int = 3) -> np.ndarray:
+        """Pick the fastest launch variant per qubit (state values unchanged);
+        returns GB/s per (qubit, variant)."""
+        nv = lib().qcs_autotune_variants()
+        out = np.zeros(self.NrQubits * nv)
+        check(lib().qcs_autotune(self._h, reps, _dptr(out)))
+        return out.reshape(self.NrQubits, nv)
+
     # ---- timing ------------------------------------------------------------------
     def synchronize(self):
         check(lib().qcs_synchronize(self._h))
