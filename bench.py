@@ -62,6 +62,31 @@ def gpu_sweep(n, steps, warmup, seed=7, tune=True):
     return ev_ms, wall, norm_err, tuned
 
 
+def gpu_qft(n, reps=3):
+    """Fused QFT (with swaps) on the n-qubit complex128 state: ms per transform."""
+    from qcsim_amd import QubitRegister
+
+    reg = QubitRegister(n)
+    reg.setToEqualSuperposition()
+    reg.qft()
+    reg.synchronize()
+    l0 = reg.launch_count()
+    reg.timer_begin()
+    for _ in range(reps):
+        reg.qft()
+    ms = reg.timer_end() / reps
+    passes = (reg.launch_count() - l0) // reps
+    reg.close()
+    gates = n + n * (n - 1) // 2 + n // 2  # the reference's H + CP + SWAP count
+    return {
+        "ms": round(ms, 3),
+        "passes": int(passes),
+        "reference_gate_count": gates,
+        "equivalent_gates_per_s": round(gates / (ms / 1e3), 1),
+        "pass_GBps": round(passes * algorithmic_bytes_1q(n) / (ms / 1e3) / 1e9, 1),
+    }
+
+
 def cpu_baseline(n, targets, seed=7):
     from oracle.oracle import OracleRegister, lib
     from qcsim_amd import gates as G
@@ -109,6 +134,7 @@ def main():
     avg_launch_s = ev_ms / 1e3 / launches
     bytes_per = algorithmic_bytes_1q(n)
     achieved = bytes_per / avg_launch_s / 1e9
+    qft = gpu_qft(n)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(n, [0, 7, 15, 22, 29] if n >= 30 else list(range(0, n, max(1, n // 5))))
@@ -141,6 +167,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "launch_variants": tuned or "default",
+        "qft": qft,
         "host_wall_s": round(wall, 4),
         "norm_drift": norm_err,
     }

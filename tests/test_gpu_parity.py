@@ -297,3 +297,60 @@ def test_random_circuit_config0():
     bad = np.nonzero(sg != so)[0]
     print(f"config0: {len(bad)} sample mismatches, min CDF margin {margin.min():.3e}")
     assert all(margin[i] < 1e-10 for i in bad)
+
+
+@pytest.mark.parametrize("sq,eq", [(0, 15), (1, 15), (2, 13), (4, 15), (3, 12), (7, 15), (0, 9), (5, 5)])
+def test_qft_fused_paths(sq, eq):
+    """Every pass type of the fused QFT (register groups, LDS chunk, tiled /
+    pair / chunk bit reversal) against the oracle's gate sequence."""
+    n = 16
+    rng = np.random.default_rng(sq * 31 + eq)
+    psi = random_state(rng, n)
+    for inverse in (False, True):
+        for swap in (True, False):
+            g, o = pair(n, psi)
+            g.qft(sq, eq, swap, inverse)
+            o.qft(sq, eq, swap, inverse)
+            err = np.max(np.abs(g.getRegisterStorage() - o.state()))
+            assert err < TOL64, (sq, eq, inverse, swap, err)
+
+
+def _product_state(rng, n):
+    """Haar U(2) on every qubit of |0...0>: psi[s] = prod_q u_q[bit_q(s)]."""
+    us = [G.haar_u2(rng) for _ in range(n)]
+    cols = [u[:, 0] for u in us]
+
+    def amp(s):
+        v = 1.0 + 0j
+        for q in range(n):
+            v *= cols[q][(s >> q) & 1]
+        return v
+
+    return us, amp
+
+
+@pytest.mark.parametrize("n", [24, 30])
+def test_qft_full_size_properties(n):
+    """Size-independent checks at full size: QFT|j> has amplitudes
+    exp(2 pi i j k / N) / sqrt(N) (the closed form sqrt(N) ifft), and
+    IQFT(QFT(psi)) = psi on a product state whose amplitudes are known."""
+    rng = np.random.default_rng(n)
+    N = 1 << n
+    r = QubitRegister(n)
+    j = int(rng.integers(N))
+    r.setToBasisState(j)
+    r.qft()
+    ks = rng.integers(N, size=32)
+    for k in ks:
+        want = np.exp(2j * np.pi * ((j * int(k)) % N) / N) / math.sqrt(N)
+        assert abs(r.getBasisStateAmplitude(int(k)) - want) < 1e-12 / math.sqrt(N) * 64
+    us, amp = _product_state(rng, n)
+    r.setToBasisState(0)
+    for q, u in enumerate(us):
+        r.ApplyGate(G.SingleQubitGate(u), q)
+    r.qft()
+    r.qft(inverse=True)
+    for s in rng.integers(N, size=64):
+        assert abs(r.getBasisStateAmplitude(int(s)) - amp(int(s))) < 1e-13
+    assert abs(r.norm2() - 1) < 1e-12
+    r.close()
