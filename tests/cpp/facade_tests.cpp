@@ -1,0 +1,313 @@
+// facade_tests.cpp - programs written against the reference's C++ API
+// (QC::QubitRegister, QC::Gates, QC::SubAlgo::QuantumFourierTransform,
+// QC::QuantumAlgorithm), compiled against include/QCSim/ and run on the GPU.
+// Each test re-expresses a check the reference's own test suite makes; the
+// reference file is named beside it. Exit status = number of failed tests.
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <functional>
+#include <iostream>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "QuantumAlgorithm.h"
+#include "QuantumFourierTransform.h"
+#include "QuantumGate.h"
+#include "QubitRegister.h"
+
+using cd = std::complex<double>;
+
+static bool approx(cd a, cd b, double e = 1e-10) { return std::abs(a.real() - b.real()) < e && std::abs(a.imag() - b.imag()) < e; }
+
+static void setRegister(QC::QubitRegister<>& reg) {
+    // 1/2|00> - i/2|01> + 1/sqrt(2)|11>  (MeasurementsTests.cpp:34-43)
+    reg.setToBasisState(0);
+    reg.setRawAmplitude(0, 0.5);
+    reg.setRawAmplitude(1, cd(0, -0.5));
+    reg.setRawAmplitude(3, 1. / std::sqrt(2.));
+    reg.Normalize();
+}
+
+// MeasurementsTests.cpp:45-167: post-measurement amplitudes for both qubits.
+static bool measurementCollapse() {
+    QC::QubitRegister reg(2);
+    std::map<size_t, size_t> counts0, counts1;
+    for (int i = 0; i < 2000; ++i) {
+        setRegister(reg);
+        const size_t s = reg.MeasureQubit(0);
+        ++counts0[s];
+        const Eigen::VectorXcd a = reg.getRegisterStorage();
+        if (s == 1) {
+            if (!approx(a(0), 0) || !approx(a(1), cd(0, -1. / std::sqrt(3.))) || !approx(a(2), 0) ||
+                !approx(a(3), std::sqrt(2.) / std::sqrt(3.)))
+                return false;
+        } else if (!approx(a(0), 1) || !approx(a(1), 0) || !approx(a(2), 0) || !approx(a(3), 0))
+            return false;
+        setRegister(reg);
+        const size_t t = reg.MeasureQubit(1);
+        ++counts1[t];
+        const Eigen::VectorXcd b = reg.getRegisterStorage();
+        if (t == 1) {
+            if (!approx(b(0), 0) || !approx(b(1), 0) || !approx(b(2), 0) || !approx(b(3), 1)) return false;
+        } else if (!approx(b(0), 0.5 * std::sqrt(2.)) || !approx(b(1), cd(0, -0.5 * std::sqrt(2.))) ||
+                   !approx(b(2), 0) || !approx(b(3), 0))
+            return false;
+    }
+    // P(q0 = 1) = 3/4, P(q1 = 1) = 1/2 (MeasurementsTests.cpp:98, 161)
+    return std::abs(counts0[1] / 2000. - 0.75) < 0.05 && std::abs(counts1[1] / 2000. - 0.5) < 0.05;
+}
+
+// Tests.cpp:665-723
+static bool expectationValues() {
+    QC::QubitRegister<> reg(3);
+    if (!approx(reg.ExpectationValue({}), 1.0, 1e-6)) return false;
+    QC::Gates::PauliZGate<> z;
+    std::vector<QC::Gates::AppliedGate<>> z0{QC::Gates::AppliedGate<>(z.getRawOperatorMatrix(), 0)};
+    if (!approx(reg.ExpectationValue(z0), 1., 1e-6)) return false;
+    QC::Gates::PauliXGate<> x;
+    reg.ApplyGate(x, 0);
+    if (!approx(reg.ExpectationValue(z0), -1., 1e-6)) return false;
+    QC::Gates::HadamardGate<> h;
+    reg.ApplyGate(h, 0);
+    if (!approx(reg.ExpectationValue(z0), 0., 1e-6)) return false;
+    reg.setToBasisState(0);
+    reg.ApplyGate(h, 1);
+    std::vector<QC::Gates::AppliedGate<>> z1{QC::Gates::AppliedGate<>(z.getRawOperatorMatrix(), 1)};
+    return approx(reg.ExpectationValue(z1), 0., 1e-6);
+}
+
+// GatesTests.cpp:11-225: every gate matrix is unitary.
+static bool gatesUnitary() {
+    auto unitary = [](const Eigen::MatrixXcd& m) {
+        const Eigen::MatrixXcd r = m.adjoint() * m;
+        for (Eigen::Index i = 0; i < r.rows(); ++i)
+            for (Eigen::Index j = 0; j < r.cols(); ++j)
+                if (!approx(r(i, j), i == j ? 1. : 0.)) return false;
+        return true;
+    };
+    using namespace QC::Gates;
+    std::vector<Eigen::MatrixXcd> ms = {
+        HadamardGate<>().getRawOperatorMatrix(), HyGate<>().getRawOperatorMatrix(), SGate<>().getRawOperatorMatrix(),
+        SDGGate<>().getRawOperatorMatrix(), TGate<>().getRawOperatorMatrix(), TDGGate<>().getRawOperatorMatrix(),
+        PhaseShiftGate<>(0.3).getRawOperatorMatrix(), PauliXGate<>().getRawOperatorMatrix(),
+        PauliYGate<>().getRawOperatorMatrix(), PauliZGate<>().getRawOperatorMatrix(),
+        SquareRootNOTGate<>().getRawOperatorMatrix(), SquareRootNOTDagGate<>().getRawOperatorMatrix(),
+        SplitterGate<>().getRawOperatorMatrix(), RxGate<>(0.7).getRawOperatorMatrix(), RyGate<>(0.7).getRawOperatorMatrix(),
+        RzGate<>(0.7).getRawOperatorMatrix(), UGate<>(0.1, 0.2, 0.3, 0.4).getRawOperatorMatrix(),
+        SwapGate<>().getRawOperatorMatrix(), iSwapGate<>().getRawOperatorMatrix(), iSwapDagGate<>().getRawOperatorMatrix(),
+        DecrementGate<>().getRawOperatorMatrix(), CNOTGate<>().getRawOperatorMatrix(),
+        ControlledYGate<>().getRawOperatorMatrix(), ControlledZGate<>().getRawOperatorMatrix(),
+        ControlledHadamardGate<>().getRawOperatorMatrix(), ControlledSquareRootNOTGate<>().getRawOperatorMatrix(),
+        ControlledSquareRootNOTDagGate<>().getRawOperatorMatrix(), ControlledPhaseGate<>().getRawOperatorMatrix(),
+        ControlledPhaseShiftGate<>(0.4).getRawOperatorMatrix(), ControlledUGate<>(0.1, 0.2, 0.3, 0.4).getRawOperatorMatrix(),
+        ControlledRxGate<>(0.5).getRawOperatorMatrix(), ControlledRyGate<>(0.5).getRawOperatorMatrix(),
+        ControlledRzGate<>(0.5).getRawOperatorMatrix(), ToffoliGate<>().getRawOperatorMatrix(),
+        FredkinGate<>().getRawOperatorMatrix(), CCZGate<>().getRawOperatorMatrix()};
+    for (const auto& m : ms)
+        if (!unitary(m)) return false;
+    return true;
+}
+
+// Bell and GHZ states, CNOT / Toffoli truth tables (closed forms).
+static bool entanglementAndTruthTables() {
+    QC::QubitRegister<> reg(3);
+    QC::Gates::HadamardGate<> h;
+    QC::Gates::CNOTGate<> cnot;
+    reg.ApplyGate(h, 0);
+    reg.ApplyGate(cnot, 1, 0);
+    reg.ApplyGate(cnot, 2, 1);
+    const auto& v = reg.getRegisterStorage();
+    const double s = 1. / std::sqrt(2.);
+    for (size_t i = 0; i < 8; ++i)
+        if (!approx(v(i), (i == 0 || i == 7) ? s : 0.)) return false;
+    QC::Gates::ToffoliGate<> tof;
+    for (size_t b = 0; b < 8; ++b) {
+        reg.setToBasisState(b);
+        reg.ApplyGate(tof, 0, 1, 2);
+        const size_t want = ((b & 6) == 6) ? (b ^ 1) : b;
+        if (!approx(reg.getBasisStateAmplitude(want), 1.)) return false;
+    }
+    return true;
+}
+
+// QFT = sqrt(N) * inverse DFT, IQFT undoes it; the gate-recorded QFT
+// (while ComputeStart is active) uncomputes (QuantumFourierTransform.h:35-87).
+static bool qftClosedForm() {
+    const size_t n = 6, N = 64;
+    Eigen::VectorXcd x(N);
+    for (size_t i = 0; i < N; ++i) x(i) = cd(std::sin(0.3 * i + 1), std::cos(0.7 * i * i));
+    QC::QubitRegister<> reg(n);
+    reg.setRegisterStorage(x);
+    const Eigen::VectorXcd in = reg.getRegisterStorage();  // normalised
+    QC::SubAlgo::QuantumFourierTransform<> qft(n);
+    qft.QFT(reg);
+    const auto& out = reg.getRegisterStorage();
+    for (size_t k = 0; k < N; ++k) {
+        cd acc = 0;
+        for (size_t j = 0; j < N; ++j) acc += in(j) * std::polar(1., 2 * M_PI * double(j * k % N) / N);
+        if (!approx(out(k), acc / std::sqrt(double(N)), 1e-12)) return false;
+    }
+    qft.IQFT(reg);
+    for (size_t k = 0; k < N; ++k)
+        if (!approx(reg.getRegisterStorage()(k), in(k), 1e-12)) return false;
+    reg.ComputeStart();
+    qft.QFT(reg, false);
+    reg.ComputeEnd();
+    reg.Uncompute();
+    for (size_t k = 0; k < N; ++k)
+        if (!approx(reg.getRegisterStorage()(k), in(k), 1e-12)) return false;
+    return true;
+}
+
+// QFT adder (the Draper construction exercised by AdderTests.cpp:213-327):
+// |a>|b> -> |a>|a+b mod 2^m> through QFT on b, controlled phases from a's
+// bits, IQFT. The outcome is deterministic.
+static bool qftAdder() {
+    const size_t m = 4;  // a on qubits 0..3, b on 4..7
+    QC::Gates::ControlledPhaseShiftGate<> cps;
+    QC::Gates::PauliXGate<> x;
+    for (size_t a = 0; a < 16; a += 3)
+        for (size_t b = 0; b < 16; b += 5) {
+            QC::QubitRegister<> reg(2 * m);
+            for (size_t i = 0; i < m; ++i) {
+                if (a >> i & 1) reg.ApplyGate(x, i);
+                if (b >> i & 1) reg.ApplyGate(x, m + i);
+            }
+            QC::SubAlgo::QuantumFourierTransform<> qft(2 * m, m, 2 * m - 1);
+            qft.QFT(reg);  // b -> (1/sqrt N) sum_k e^{2 pi i b k / N} |k>
+            for (size_t j = 0; j < m; ++j)      // qubit j of k
+                for (size_t l = 0; l + j < m; ++l) {  // bit l of a
+                    cps.SetPhaseShift(2 * M_PI * double(1ULL << (l + j)) / double(1ULL << m));
+                    reg.ApplyGate(cps, m + j, l);
+                }
+            qft.IQFT(reg);
+            const size_t got = reg.MeasureAll();
+            if (got != (a | (((a + b) % 16) << m))) return false;
+        }
+    return true;
+}
+
+// Bernstein-Vazirani as a QuantumAlgorithm (the deterministic-answer tests of
+// Tests.cpp:333-617): H on all, phase oracle (-1)^{s.x}, H on all -> s.
+class BV : public QC::QuantumAlgorithm<> {
+public:
+    BV(size_t n, size_t s) : QC::QuantumAlgorithm<>(n), s_(s) {}
+    size_t Execute() override {
+        QC::Gates::HadamardGate<> h;
+        QC::Gates::PauliZGate<> z;
+        for (size_t q = 0; q < getNrQubits(); ++q) ApplyGate(h, q);
+        for (size_t q = 0; q < getNrQubits(); ++q)
+            if (s_ >> q & 1) ApplyGate(z, q);
+        for (size_t q = 0; q < getNrQubits(); ++q) ApplyGate(h, q);
+        return Measure();
+    }
+
+private:
+    size_t s_;
+};
+
+static bool bernsteinVazirani() {
+    for (size_t s : {size_t(0), size_t(5), size_t(1000), size_t(4095)}) {
+        BV bv(12, s);
+        if (bv.Execute() != s) return false;
+    }
+    return true;
+}
+
+// CheckQubits messages (QubitRegister.h:677-690).
+static bool invalidArguments() {
+    QC::QubitRegister<> reg(3);
+    auto expect = [](const std::function<void()>& f, const std::string& msg) {
+        try {
+            f();
+        } catch (const std::invalid_argument& e) {
+            return msg == e.what();
+        }
+        return false;
+    };
+    QC::Gates::HadamardGate<> h;
+    QC::Gates::CNOTGate<> cnot;
+    QC::Gates::ToffoliGate<> tof;
+    return expect([&] { reg.ApplyGate(h, 3); }, "Qubit number is too high") &&
+           expect([&] { reg.ApplyGate(cnot, 0, 3); }, "Controlling qubit number is too high") &&
+           expect([&] { reg.ApplyGate(cnot, 1, 1); }, "Qubit and controlling qubit are the same") &&
+           expect([&] { reg.ApplyGate(tof, 0, 1, 1); }, "Qubits must be different");
+}
+
+// SaveState / RestoreState, Clone, swap semantics, setRegisterStorage.
+static bool stateManagement() {
+    QC::QubitRegister<> reg(4);
+    QC::Gates::HadamardGate<> h;
+    reg.ApplyGate(h, 2);
+    reg.SaveState();
+    reg.ApplyGate(h, 0);
+    reg.RestoreState();
+    if (!approx(reg.getBasisStateAmplitude(4), 1. / std::sqrt(2.))) return false;
+    auto c = reg.Clone();
+    if (!approx(c->getBasisStateAmplitude(4), 1. / std::sqrt(2.))) return false;
+    Eigen::VectorXcd v = Eigen::VectorXcd::Zero(16);
+    v(3) = 1;
+    reg.setRegisterStorageFastNoNormalize(v);  // v receives the old state
+    if (!approx(reg.getBasisStateAmplitude(3), 1.) || !approx(v(4), 1. / std::sqrt(2.))) return false;
+    Eigen::VectorXcd w = Eigen::VectorXcd::Zero(16);
+    w(7) = 2;
+    QC::QubitRegister<> r2(4, w);  // the (N, v) ctor takes v's contents
+    if (!approx(r2.getBasisStateAmplitude(7), 2.) || w.size() != 0) return false;
+    reg.setRegisterStorage(v);  // normalises
+    if (std::abs(reg.stateFidelity(v) - 1.) > 1e-12) return false;
+    reg.setToEqualSuperposition();
+    return std::abs(reg.GetQubitProbability(3) - 0.5) < 1e-12;
+}
+
+// RepeatedMeasure statistics and the diagonal ApplyOperatorMatrix path.
+static bool samplingAndOperators() {
+    QC::QubitRegister<> reg(2);
+    QC::Gates::HadamardGate<> h;
+    reg.ApplyGate(h, 0);
+    auto counts = reg.RepeatedMeasure(20000);
+    if (counts.size() != 2 || std::abs(double(counts[1]) / 20000 - 0.5) > 0.02) return false;
+    auto r = reg.RepeatedMeasure(0, 0, 10);
+    size_t tot = 0;
+    for (auto& kv : r) tot += kv.second;
+    if (tot != 10) return false;
+    Eigen::MatrixXcd d = Eigen::MatrixXcd::Identity(4, 4);
+    d(1, 1) = cd(0, 1);
+    reg.ApplyOperatorMatrix(d);
+    if (!approx(reg.getBasisStateAmplitude(1), cd(0, 1. / std::sqrt(2.)))) return false;
+    Eigen::MatrixXcd p = Eigen::MatrixXcd::Zero(4, 4);  // permutation 0<->1
+    p(0, 1) = p(1, 0) = p(2, 2) = p(3, 3) = 1;
+    reg.ApplyOperatorMatrix(p);
+    return approx(reg.getBasisStateAmplitude(0), cd(0, 1. / std::sqrt(2.)));
+}
+
+int main() {
+    const std::vector<std::pair<const char*, bool (*)()>> tests = {
+        {"measurementCollapse", measurementCollapse},
+        {"expectationValues", expectationValues},
+        {"gatesUnitary", gatesUnitary},
+        {"entanglementAndTruthTables", entanglementAndTruthTables},
+        {"qftClosedForm", qftClosedForm},
+        {"qftAdder", qftAdder},
+        {"bernsteinVazirani", bernsteinVazirani},
+        {"invalidArguments", invalidArguments},
+        {"stateManagement", stateManagement},
+        {"samplingAndOperators", samplingAndOperators},
+    };
+    int failed = 0;
+    for (const auto& t : tests) {
+        std::cout << t.first << " ... " << std::flush;
+        bool ok = false;
+        try {
+            ok = t.second();
+        } catch (const std::exception& e) {
+            std::cout << "exception: " << e.what() << " ";
+        }
+        std::cout << (ok ? "PASS" : "FAIL") << std::endl;
+        failed += ok ? 0 : 1;
+    }
+    std::cout << (failed ? "FAILED " : "ALL PASSED ") << failed << std::endl;
+    return failed;
+}
