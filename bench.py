@@ -87,6 +87,87 @@ def gpu_qft(n, reps=3):
     }
 
 
+def random_circuit_c1(n, layers, seed=11):
+    """Config 1 circuit: per layer a Haar U(2) on every qubit, then a random
+    perfect pairing with CNOT, CZ or controlled-Haar-U(2) (random orientation)."""
+    from qcsim_amd import gates as G
+
+    rng = np.random.default_rng(seed)
+    seq = []
+    for _ in range(layers):
+        for q in range(n):
+            seq.append((G.SingleQubitGate(G.haar_u2(rng)), [q]))
+        perm = rng.permutation(n)
+        for a, b in zip(perm[0::2], perm[1::2]):
+            c = int(rng.integers(3))
+            if c == 0:
+                g = G.CNOTGate()
+            elif c == 1:
+                g = G.ControlledZGate()
+            else:
+                g = G.TwoQubitsControlledGate()
+                g.SetOperation(G.haar_u2(rng))
+            seq.append((g, [int(a), int(b)]))
+    return seq
+
+
+def gate_bytes(gate, n, amp=16):
+    """Algorithmic bytes 2 s 2^n f, f = fraction of amplitudes changed."""
+    from qcsim_amd import gates as G
+
+    t = gate.traits
+    k = gate.getQubitsNumber()
+    m = gate.getRawOperatorMatrix()
+    f = 1.0
+    if k == 1 and (t & G.T_DIAGONAL) and m[0, 0] == 1:
+        f = 0.5
+    elif k == 2 and (t & G.T_CONTROLLED):
+        f = 0.25 if ((t & G.T_DIAGONAL) and m[2, 2] == 1) else 0.5
+    elif k == 2 and (t & (G.T_SWAP | G.T_ISWAP | G.T_ISWAPDAG)):
+        f = 0.5
+    return 2 * amp * (1 << n) * f
+
+
+def gpu_circuit(n, layers, widths=(0, 4, 5)):
+    """The config 1 random circuit, unfused and fused; one run each after a
+    warm-up on a smaller circuit of the same kind."""
+    from qcsim_amd import QubitRegister
+
+    seq = random_circuit_c1(n, layers)
+    total_bytes = sum(gate_bytes(g, n) for g, _ in seq)
+    reg = QubitRegister(n)
+    out = {"qubits": n, "layers": layers, "gates": len(seq)}
+    states = {}
+    for w in widths:
+        reg.setToBasisState(0)
+        reg.apply_circuit(seq[: 2 * (n + n // 2)], fuse_width=w)  # warm-up
+        reg.setToBasisState(0)
+        reg.synchronize()
+        l0 = reg.launch_count()
+        reg.timer_begin()
+        reg.apply_circuit(seq, fuse_width=w)
+        ms = reg.timer_end()
+        launches = reg.launch_count() - l0
+        key = "unfused" if w == 0 else f"fused{w}"
+        out[key] = {
+            "s": round(ms / 1e3, 3),
+            "gates_per_s": round(len(seq) / (ms / 1e3), 1),
+            "launches": int(launches),
+            "algorithmic_GBps": round(total_bytes / (ms / 1e3) / 1e9, 1),
+        }
+        states[key] = reg.Clone()
+    if "unfused" in states:
+        for k, r in states.items():
+            if k != "unfused":
+                mx, ip = r.compare(states["unfused"])
+                out[k]["max_abs_vs_unfused"] = mx
+                out[k]["infidelity_vs_unfused"] = max(0.0, 1 - abs(ip) ** 2)
+    for r in states.values():
+        r.close()
+    reg.close()
+    return out
+
+
 def cpu_baseline(n, targets, seed=7):
     from oracle.oracle import OracleRegister, lib
     from qcsim_amd import gates as G
@@ -119,6 +200,7 @@ def main():
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-tune", action="store_true", help="use the default launch variants")
+    ap.add_argument("--circuit-layers", type=int, default=60, help="config 1 circuit depth (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -135,6 +217,7 @@ def main():
     bytes_per = algorithmic_bytes_1q(n)
     achieved = bytes_per / avg_launch_s / 1e9
     qft = gpu_qft(n)
+    circuit = gpu_circuit(n, args.circuit_layers) if args.circuit_layers > 0 else None
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(n, [0, 7, 15, 22, 29] if n >= 30 else list(range(0, n, max(1, n // 5))))
@@ -168,6 +251,7 @@ def main():
         "cpu_baseline": cpu,
         "launch_variants": tuned or "default",
         "qft": qft,
+        "circuit": circuit,
         "host_wall_s": round(wall, 4),
         "norm_drift": norm_err,
     }

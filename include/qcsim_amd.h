@@ -50,12 +50,13 @@ extern "C" {
 
 typedef struct qcs_register qcs_register;
 
-/* One gate of a circuit batch (qcs_apply_circuit). */
+/* One gate of a circuit batch (qcs_apply_circuit, qcs_fuse). */
+#define QCS_GATE_DENSE 0x1u /* flags: dense block on qubits[0..k)           */
 typedef struct qcs_gate {
     uint32_t k;        /* 1..3: reference ApplyGate; 4..5: dense block      */
-    uint32_t traits;   /* QCS_TRAIT_* (ignored for k > 3)                    */
-    uint32_t qubits[5];/* k<=3: qubit, c1, c2; k>3: local bit i -> qubits[i] */
-    uint32_t pad;
+    uint32_t traits;   /* QCS_TRAIT_* (ignored for dense blocks)             */
+    uint32_t qubits[5];/* ApplyGate: qubit, c1, c2; dense: bit i -> qubits[i]*/
+    uint32_t flags;    /* QCS_GATE_DENSE; k > 3 is always dense              */
     double   mat[2 * 32 * 32]; /* row-major interleaved, first 2*4^k used   */
 } qcs_gate;
 
@@ -113,6 +114,14 @@ int qcs_apply_dense(qcs_register* r, uint32_t k, const uint32_t* qubits,
  * runs the fusion pass first (blocks of at most fuse_width qubits). */
 int qcs_apply_circuit(qcs_register* r, const qcs_gate* gates, uint32_t n,
                       uint32_t fuse_width);
+
+/* The fusion pass alone (host only, no device): rewrites `gates` into at
+ * most n blocks of <= width qubits whose product equals the sequence, each
+ * gate taken with the matrix its traits make the reference apply. Blocks of
+ * a single gate keep that gate unchanged; merged blocks are dense
+ * (QCS_GATE_DENSE, qubits ascending). No counterpart in the reference. */
+int qcs_fuse(const qcs_gate* gates, uint32_t n, uint32_t width,
+             qcs_gate* out, uint32_t* n_out);
 
 /* Diagonal operator on the whole register: psi[s] *= diag[s]
  * (ApplyOperatorMatrix with a diagonal matrix, QubitRegister.h:499-505). */

@@ -13,6 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libqcsim_amd.so")
 
 QCS_OK, QCS_EINVAL, QCS_ENOMEM, QCS_EDEVICE = 0, 1, 2, 3
+QCS_GATE_DENSE = 0x1
 QCS_COMPLEX128, QCS_COMPLEX64 = 0, 1
 
 # Every symbol include/qcsim_amd.h declares, with its ctypes signature.
@@ -30,7 +31,7 @@ class QcsGate(C.Structure):
         ("k", C.c_uint32),
         ("traits", C.c_uint32),
         ("qubits", C.c_uint32 * 5),
-        ("pad", C.c_uint32),
+        ("flags", C.c_uint32),
         ("mat", C.c_double * (2 * 32 * 32)),
     ]
 
@@ -53,6 +54,7 @@ SIGNATURES = {
     "qcs_apply_dense": (_I, [_P, _U32, _PU32, _PD]),
     "qcs_apply_circuit": (_I, [_P, C.POINTER(QcsGate), _U32, _U32]),
     "qcs_apply_diagonal": (_I, [_P, _PD, _U64]),
+    "qcs_fuse": (_I, [C.POINTER(QcsGate), _U32, _U32, C.POINTER(QcsGate), C.POINTER(_U32)]),
     "qcs_qft": (_I, [_P, _U32, _U32, _I, _I]),
     "qcs_measure_all": (_I, [_P, _D, _PU64]),
     "qcs_measure": (_I, [_P, _U32, _U32, _D, _PU64]),

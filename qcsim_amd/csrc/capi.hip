@@ -45,6 +45,7 @@ int check_launch(qcs_register* r, const char* what) {
 }
 
 int qft_sequence(qcs_register* r, uint32_t sq, uint32_t eq, int do_swap, int inverse);
+int fuse(const qcs_gate* gates, uint32_t n, uint32_t width, std::vector<qcs_gate>& out);
 
 // conversions for complex64 registers
 __global__ void k_d2f(const double2* __restrict__ in, float2* __restrict__ out, uint64_t n) {
@@ -107,7 +108,7 @@ int qcs_create(uint32_t nqubits, int precision, qcs_register** out) {
         hipGetLastError();
         return fail(QCS_ENOMEM, "hipMalloc of the state failed");
     }
-    r->mat_bytes = sizeof(double) * 2 * 32 * 32;
+    r->mat_bytes = sizeof(double) * 64 * 64;  // real embedding of a 32x32 complex block
     if (hipMalloc(reinterpret_cast<void**>(&r->red), sizeof(double) * (kRedBlocks * 3 + 64)) != hipSuccess ||
         hipMalloc(reinterpret_cast<void**>(&r->chunk), sizeof(double) * nchunks) != hipSuccess ||
         hipMalloc(reinterpret_cast<void**>(&r->dout), sizeof(uint64_t) * 64) != hipSuccess ||
@@ -362,11 +363,23 @@ int qcs_apply_dense(qcs_register* r, uint32_t k, const uint32_t* qubits, const d
 
 int qcs_apply_circuit(qcs_register* r, const qcs_gate* gates, uint32_t n, uint32_t fuse_width) {
     QCS_REQUIRE(r && (gates || n == 0), "null argument");
-    (void)fuse_width;
-    for (uint32_t i = 0; i < n; ++i) {
-        const qcs_gate& g = gates[i];
-        const int e = g.k <= 3 ? qcs_apply_gate(r, g.k, g.traits, g.qubits[0], g.qubits[1], g.qubits[2], g.mat)
-                               : qcs_apply_dense(r, g.k, g.qubits, g.mat);
+    std::vector<qcs_gate> fused;
+    const qcs_gate* seq = gates;
+    uint32_t count = n;
+    if (fuse_width >= 2) {
+        for (uint32_t i = 0; i < n; ++i)
+            for (uint32_t j = 0; j < gates[i].k && j < 5; ++j)
+                QCS_REQUIRE(gates[i].qubits[j] < r->n, "gate %u: qubit out of range", i);
+        const int e = fuse(gates, n, fuse_width, fused);
+        if (e) return e;
+        seq = fused.data();
+        count = uint32_t(fused.size());
+    }
+    for (uint32_t i = 0; i < count; ++i) {
+        const qcs_gate& g = seq[i];
+        const bool dense = (g.flags & QCS_GATE_DENSE) || g.k > 3;
+        const int e = dense ? qcs_apply_dense(r, g.k, g.qubits, g.mat)
+                            : qcs_apply_gate(r, g.k, g.traits, g.qubits[0], g.qubits[1], g.qubits[2], g.mat);
         if (e) return e;
     }
     return QCS_OK;

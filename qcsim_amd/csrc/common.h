@@ -115,6 +115,7 @@ struct qcs_register {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     uint64_t launches = 0;
+    bool dense_no_mfma = false;  // route dense blocks to the non-MFMA kernel
 
     size_t amp_bytes() const { return prec == QCS_COMPLEX128 ? 16 : 8; }
 };

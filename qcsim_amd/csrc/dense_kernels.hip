@@ -3,8 +3,14 @@
 // A block is a 2^k x 2^k complex matrix acting on the amplitudes that differ
 // only in the block's k qubits (local bit i <-> qubits[i]); this is the
 // Kronecker embedding of SimpleGates.h:211-232/311-333/411-434 generalised
-// to k bits. k <= 3 uses the gate kernels; k = 4, 5 stage the matrix in LDS
-// and keep the orbit's 2^k amplitudes in registers.
+// to k bits. k <= 3 uses the gate kernels. k = 4, 5 run on the matrix cores
+// (k_dense_mfma): the complex block M becomes the real 2D x 2D matrix
+// R = [[Re M, -Im M], [Im M, Re M]] acting on [Re x; Im x]; a workgroup
+// stages 64 orbits (2D x 64 reals) in LDS and each wave multiplies 16 of
+// them with v_mfma_f64_16x16x4f64 (complex128) or v_mfma_f32_16x16x4f32
+// (complex64), the R fragments held in registers. 8 * 2^k FLOP per
+// amplitude. k_dense (matrix in LDS, orbit in registers, no MFMA) is the
+// fallback for states too small to tile.
 #include <algorithm>
 #include <vector>
 
@@ -56,8 +62,146 @@ __global__ __launch_bounds__(256) void k_dense(cv<T>* __restrict__ psi, uint64_t
     }
 }
 
+typedef double mf_d4 __attribute__((ext_vector_type(4)));
+typedef float mf_f4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+    using acc_t = mf_d4;
+    __device__ static acc_t step(double a, double b, acc_t c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    // output element (lane, reg) -> row within the 16x16 tile
+    __device__ static int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+template <> struct Mfma<float> {
+    using acc_t = mf_f4;
+    __device__ static acc_t step(float a, float b, acc_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    __device__ static int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+// Tile geometry: the block's K qubits plus the 6 lowest other bits form the
+// 11 (K=5) / 10 (K=4) "tile positions", ascending. Element e of a tile
+// deposits its bits into those positions, so consecutive lanes touch
+// consecutive addresses wherever the positions are contiguous from bit 0.
+struct TileGeom {
+    uint32_t npos;
+    uint8_t pos[12];
+    uint8_t role[12];  // < 0x80: local block bit; 0x80 | j: orbit bit j
+};
+
+template <typename T, int K>
+__global__ __launch_bounds__(256) void k_dense_mfma(cv<T>* __restrict__ psi, uint64_t ntiles, TileGeom tg,
+                                                    const T* __restrict__ R) {
+    constexpr int D = 1 << K, RD = 2 * D, NB = 64, E = D * NB, EPT = E / 256, LDX = NB + 2;
+    constexpr int NT = RD / 16, NS = RD / 4;
+    __shared__ __attribute__((aligned(16))) T sX[RD * LDX];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    // per-thread element map (the same for every tile)
+    uint64_t off[EPT];
+    int row[EPT], col[EPT];
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) {
+        const uint32_t e = uint32_t(tid + 256 * j);
+        uint64_t o = 0;
+        int c = 0, ol = 0;
+        for (uint32_t p = 0; p < tg.npos; ++p) {
+            const uint32_t bit = (e >> p) & 1u;
+            o |= uint64_t(bit) << tg.pos[p];
+            const uint8_t rl = tg.role[p];
+            if (rl & 0x80) ol |= int(bit) << (rl & 0x7f);
+            else c |= int(bit) << rl;
+        }
+        off[j] = o;
+        row[j] = c;
+        col[j] = ol;
+    }
+    // R fragments: A[t][s] = R[16t + lane%16][4s + lane/16]
+    T af[NT][NS];
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int s = 0; s < NS; ++s) af[t][s] = R[(16 * t + (lane & 15)) * RD + 4 * s + (lane >> 4)];
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        uint64_t base = tile;
+        for (uint32_t p = 0; p < tg.npos; ++p) {
+            const uint64_t low = (1ull << tg.pos[p]) - 1;
+            base = ((base & ~low) << 1) | (base & low);
+        }
+#pragma unroll
+        for (int j = 0; j < EPT; ++j) {
+            const cv<T> v = ldamp<T, true>(psi + (base | off[j]));
+            sX[row[j] * LDX + col[j]] = v.x;
+            sX[(D + row[j]) * LDX + col[j]] = v.y;
+        }
+        __syncthreads();
+        typename Mfma<T>::acc_t acc[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[t] = typename Mfma<T>::acc_t{0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const T b = sX[(4 * s + (lane >> 4)) * LDX + 16 * w + (lane & 15)];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) acc[t] = Mfma<T>::step(af[t][s], b, acc[t]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) sX[(16 * t + Mfma<T>::row(lane, r)) * LDX + 16 * w + (lane & 15)] = acc[t][r];
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < EPT; ++j) {
+            cv<T> v;
+            v.x = sX[row[j] * LDX + col[j]];
+            v.y = sX[(D + row[j]) * LDX + col[j]];
+            stamp<T, true>(psi + (base | off[j]), v);
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T, int K>
+static int kdense_mfma(qcs_register* r, const uint32_t* qubits, const double* mat_host) {
+    constexpr int D = 1 << K, RD = 2 * D;
+    std::vector<T> R(size_t(RD) * RD);
+    for (int i = 0; i < D; ++i)
+        for (int j = 0; j < D; ++j) {
+            const T re = T(mat_host[2 * (i * D + j)]), im = T(mat_host[2 * (i * D + j) + 1]);
+            R[size_t(i) * RD + j] = re;
+            R[size_t(i) * RD + D + j] = -im;
+            R[size_t(D + i) * RD + j] = im;
+            R[size_t(D + i) * RD + D + j] = re;
+        }
+    if (hipMemcpyAsync(r->mat, R.data(), sizeof(T) * R.size(), hipMemcpyHostToDevice, r->stream) != hipSuccess)
+        return set_error(QCS_EDEVICE, "dense matrix upload failed");
+    TileGeom tg{};
+    uint64_t gmask = 0;
+    for (int i = 0; i < K; ++i) gmask |= 1ull << qubits[i];
+    uint32_t orb[6], no = 0;
+    for (uint32_t b = 0; no < 6; ++b)
+        if (!(gmask >> b & 1)) orb[no++] = b;
+    uint32_t all[11];
+    for (int i = 0; i < K; ++i) all[i] = qubits[i];
+    for (int j = 0; j < 6; ++j) all[K + j] = orb[j];
+    std::sort(all, all + K + 6);
+    tg.npos = K + 6;
+    for (uint32_t p = 0; p < tg.npos; ++p) {
+        tg.pos[p] = uint8_t(all[p]);
+        tg.role[p] = 0xff;
+        for (int i = 0; i < K; ++i)
+            if (qubits[i] == all[p]) tg.role[p] = uint8_t(i);
+        for (int j = 0; j < 6; ++j)
+            if (orb[j] == all[p]) tg.role[p] = uint8_t(0x80 | j);
+    }
+    const uint64_t ntiles = r->A >> (K + 6);
+    const unsigned grid = unsigned(ntiles < 4096 ? ntiles : 4096);
+    hipLaunchKernelGGL((k_dense_mfma<T, K>), dim3(grid), dim3(256), 0, r->stream, static_cast<cv<T>*>(r->psi), ntiles,
+                       tg, static_cast<const T*>(r->mat));
+    return check_launch(r, "k_dense_mfma");
+}
+
 template <typename T, int K>
 static int kdense_big(qcs_register* r, const uint32_t* qubits, const double* mat_host) {
+    if (r->n >= uint32_t(K + 6) && !r->dense_no_mfma) return kdense_mfma<T, K>(r, qubits, mat_host);
     constexpr int D = 1 << K;
     // matrix converted to T, split re[] then im[], uploaded from pageable
     // memory (the copy has consumed the host buffer when the call returns).

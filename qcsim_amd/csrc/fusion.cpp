@@ -1,0 +1,230 @@
+// fusion.cpp - gate fusion pass (host C++).
+//
+// Turns a gate sequence into blocks of at most W qubits, each applied in one
+// pass over the state. A gate may join an open block only if that block
+// comes after every block that already touches one of its qubits (so no
+// gate moves ahead of an earlier gate on a shared qubit) and the union of
+// qubits stays within W; among those it joins the block whose qubit set grows
+// least (latest on ties), else it opens a new block at the end. Gates on
+// disjoint qubits commute, so the block order is a valid reordering.
+//
+// Each gate enters the product with its *effective* matrix: the matrix that
+// the reference's trait-driven kernels apply (QubitRegisterCalculator.h:
+// 39-227, 599-639), e.g. a controlled gate contributes identity on the
+// control=0 block and only block rows/cols 2..3. For gates whose traits agree
+// with their matrix (all gate classes of the reference) this is the matrix.
+#include <algorithm>
+#include <complex>
+#include <cstring>
+#include <vector>
+
+#include "qcsim_amd.h"
+
+namespace qcs {
+
+int set_error(int code, const char* fmt, ...);
+
+using cplx = std::complex<double>;
+
+// Effective 2^k x 2^k matrix of a reference gate (k = 1..3) or a dense block.
+static std::vector<cplx> effective_matrix(const qcs_gate& g) {
+    const int k = int(g.k), D = 1 << k;
+    std::vector<cplx> full(size_t(D) * D);
+    for (int i = 0; i < D * D; ++i) full[i] = cplx(g.mat[2 * i], g.mat[2 * i + 1]);
+    if ((g.flags & QCS_GATE_DENSE) || k > 3) return full;
+    auto at = [&](int r, int c) { return full[size_t(r) * D + c]; };
+    std::vector<cplx> m(size_t(D) * D, cplx(0, 0));
+    auto set = [&](int r, int c, cplx v) { m[size_t(r) * D + c] = v; };
+    auto ident_upto = [&](int lim) {
+        for (int i = 0; i < lim; ++i) set(i, i, 1.0);
+    };
+    const uint32_t t = g.traits;
+    if (k == 1) {
+        if (t & QCS_TRAIT_DIAGONAL) {
+            set(0, 0, at(0, 0));
+            set(1, 1, at(1, 1));
+        } else if (t & QCS_TRAIT_ANTIDIAGONAL) {
+            set(0, 1, at(0, 1));
+            set(1, 0, at(1, 0));
+        } else {
+            return full;
+        }
+        return m;
+    }
+    if (k == 2) {
+        if (t & (QCS_TRAIT_SWAP | QCS_TRAIT_ISWAP | QCS_TRAIT_ISWAPDAG)) {
+            const cplx ph = (t & QCS_TRAIT_SWAP) ? cplx(1, 0) : ((t & QCS_TRAIT_ISWAP) ? cplx(0, 1) : cplx(0, -1));
+            set(0, 0, 1.0);
+            set(3, 3, 1.0);
+            set(1, 2, ph);
+            set(2, 1, ph);
+            return m;
+        }
+        if (t & QCS_TRAIT_CONTROLLED) {
+            ident_upto(2);
+            if (t & QCS_TRAIT_DIAGONAL) {
+                set(2, 2, at(2, 2));
+                set(3, 3, at(3, 3));
+            } else if (t & QCS_TRAIT_ANTIDIAGONAL) {
+                set(2, 3, at(2, 3));
+                set(3, 2, at(3, 2));
+            } else {
+                for (int r = 2; r < 4; ++r)
+                    for (int c = 2; c < 4; ++c) set(r, c, at(r, c));
+            }
+            return m;
+        }
+        return full;
+    }
+    // k == 3
+    if (t & QCS_TRAIT_SWAP) {  // Fredkin: swap local 5 <-> 6
+        for (int i = 0; i < 8; ++i)
+            if (i != 5 && i != 6) set(i, i, 1.0);
+        set(5, 6, 1.0);
+        set(6, 5, 1.0);
+        return m;
+    }
+    if (t & QCS_TRAIT_CONTROLLED) {
+        if (t & QCS_TRAIT_CTRL_Q1) {
+            ident_upto(6);
+            if (t & QCS_TRAIT_DIAGONAL) {
+                set(6, 6, at(6, 6));
+                set(7, 7, at(7, 7));
+            } else if (t & QCS_TRAIT_ANTIDIAGONAL) {
+                set(6, 7, at(6, 7));
+                set(7, 6, at(7, 6));
+            } else {
+                for (int r = 6; r < 8; ++r)
+                    for (int c = 6; c < 8; ++c) set(r, c, at(r, c));
+            }
+            return m;
+        }
+        ident_upto(4);
+        for (int r = 4; r < 8; ++r)
+            for (int c = 4; c < 8; ++c) set(r, c, at(r, c));
+        return m;
+    }
+    return full;
+}
+
+static int gate_qubits(const qcs_gate& g, uint32_t* q) {
+    for (uint32_t i = 0; i < g.k; ++i) q[i] = g.qubits[i];
+    return int(g.k);
+}
+
+struct Block {
+    std::vector<uint32_t> gates;  // indices into the input, in order
+    uint64_t mask64 = 0;          // qubits touched (bit per qubit)
+};
+
+static int popc(uint64_t x) { return __builtin_popcountll(x); }
+
+// M <- G * M where G acts on `gq` (local positions within the block's sorted
+// qubit list `bq`) of a w-qubit block.
+static void left_multiply(std::vector<cplx>& M, int w, const std::vector<uint32_t>& bq, const qcs_gate& g) {
+    const std::vector<cplx> G = effective_matrix(g);
+    uint32_t gq[5];
+    const int k = gate_qubits(g, gq);
+    int pos[5];
+    for (int i = 0; i < k; ++i) pos[i] = int(std::find(bq.begin(), bq.end(), gq[i]) - bq.begin());
+    const int W = 1 << w, K = 1 << k;
+    uint32_t gmask = 0;
+    for (int i = 0; i < k; ++i) gmask |= 1u << pos[i];
+    std::vector<cplx> out(size_t(W) * W, cplx(0, 0));
+    auto local = [&](int s) {
+        int l = 0;
+        for (int i = 0; i < k; ++i) l |= ((s >> pos[i]) & 1) << i;
+        return l;
+    };
+    auto deposit = [&](int base, int l) {
+        int s = base & ~int(gmask);
+        for (int i = 0; i < k; ++i) s |= ((l >> i) & 1) << pos[i];
+        return s;
+    };
+    for (int r = 0; r < W; ++r) {
+        const int lr = local(r);
+        for (int c = 0; c < K; ++c) {
+            const cplx gv = G[size_t(lr) * K + c];
+            if (gv == cplx(0, 0)) continue;
+            const int src = deposit(r, c);
+            for (int j = 0; j < W; ++j) out[size_t(r) * W + j] += gv * M[size_t(src) * W + j];
+        }
+    }
+    M.swap(out);
+}
+
+int fuse(const qcs_gate* gates, uint32_t n, uint32_t width, std::vector<qcs_gate>& out) {
+    if (width < 1 || width > 5) return set_error(QCS_EINVAL, "fusion width must be 1..5");
+    std::vector<Block> blocks;
+    for (uint32_t i = 0; i < n; ++i) {
+        const qcs_gate& g = gates[i];
+        if (g.k < 1 || g.k > 5) return set_error(QCS_EINVAL, "gate %u: arity must be 1..5", i);
+        uint64_t gm = 0;
+        for (uint32_t j = 0; j < g.k; ++j) {
+            if (g.qubits[j] >= 64) return set_error(QCS_EINVAL, "gate %u: qubit out of range", i);
+            gm |= 1ull << g.qubits[j];
+        }
+        if (popc(gm) != int(g.k)) return set_error(QCS_EINVAL, "gate %u: qubits must be different", i);
+        // last block touching any of the gate's qubits
+        int last = -1;
+        for (int b = int(blocks.size()) - 1; b >= 0; --b)
+            if (blocks[b].mask64 & gm) {
+                last = b;
+                break;
+            }
+        int best = -1, best_w = 99;
+        const int lo = std::max(0, last);
+        const int search_from = std::max(lo, int(blocks.size()) - 64);
+        for (int b = int(blocks.size()) - 1; b >= search_from; --b) {
+            const int w = popc(blocks[b].mask64 | gm);
+            if (w <= int(width) && w < best_w) {
+                best = b;
+                best_w = w;
+            }
+        }
+        if (best < 0 || (g.k > width)) {
+            blocks.push_back(Block{});
+            best = int(blocks.size()) - 1;
+        }
+        blocks[best].gates.push_back(i);
+        blocks[best].mask64 |= gm;
+    }
+    out.clear();
+    out.reserve(blocks.size());
+    for (const Block& b : blocks) {
+        if (b.gates.size() == 1) {
+            out.push_back(gates[b.gates[0]]);
+            continue;
+        }
+        std::vector<uint32_t> bq;
+        for (uint32_t q = 0; q < 64; ++q)
+            if (b.mask64 >> q & 1) bq.push_back(q);
+        const int w = int(bq.size()), W = 1 << w;
+        std::vector<cplx> M(size_t(W) * W, cplx(0, 0));
+        for (int d = 0; d < W; ++d) M[size_t(d) * W + d] = 1.0;
+        for (uint32_t gi : b.gates) left_multiply(M, w, bq, gates[gi]);
+        qcs_gate fg;
+        std::memset(&fg, 0, sizeof fg);
+        fg.k = uint32_t(w);
+        fg.flags = QCS_GATE_DENSE;
+        for (int j = 0; j < w; ++j) fg.qubits[j] = bq[j];
+        for (int e = 0; e < W * W; ++e) {
+            fg.mat[2 * e] = M[e].real();
+            fg.mat[2 * e + 1] = M[e].imag();
+        }
+        out.push_back(fg);
+    }
+    return QCS_OK;
+}
+
+}  // namespace qcs
+
+extern "C" int qcs_fuse(const qcs_gate* gates, uint32_t n, uint32_t width, qcs_gate* out, uint32_t* n_out) {
+    if ((!gates && n) || !out || !n_out) return qcs::set_error(QCS_EINVAL, "null argument");
+    std::vector<qcs_gate> res;
+    const int e = qcs::fuse(gates, n, width, res);
+    if (e) return e;
+    std::memcpy(out, res.data(), res.size() * sizeof(qcs_gate));
+    *n_out = uint32_t(res.size());
+    return QCS_OK;
+}

@@ -354,3 +354,44 @@ def test_qft_full_size_properties(n):
         assert abs(r.getBasisStateAmplitude(int(s)) - amp(int(s))) < 1e-13
     assert abs(r.norm2() - 1) < 1e-12
     r.close()
+
+
+@pytest.mark.parametrize("precision,tol", [("c128", TOL64), ("c64", 5e-6)])
+@pytest.mark.parametrize("n,k", [(11, 4), (11, 5), (16, 5), (16, 4), (12, 3)])
+def test_dense_blocks_mfma(precision, tol, n, k):
+    """Dense k-qubit blocks through the MFMA kernel (k = 4, 5 when n >= k + 6)
+    at low, high, mixed and reversed qubit orders."""
+    from refmath import random_unitary
+
+    rng = np.random.default_rng(100 * n + k)
+    choices = [list(range(k)), list(range(n - k, n)), list(range(k))[::-1],
+               [int(x) for x in rng.choice(n, size=k, replace=False)]]
+    for qs in choices:
+        m = random_unitary(rng, 1 << k)
+        psi = random_state(rng, n)
+        g, o = pair(n, psi, precision)
+        g.apply_dense(qs, m)
+        o.apply_dense(qs, m)
+        err = np.max(np.abs(g.getRegisterStorage() - o.state()))
+        assert err < tol, (qs, err)
+
+
+@pytest.mark.parametrize("precision,tol", [("c128", TOL64), ("c64", 2e-5)])
+@pytest.mark.parametrize("width", [2, 3, 4, 5])
+def test_fused_circuit(precision, tol, width):
+    """qcs_apply_circuit with fusion against the oracle's gate-by-gate run."""
+    n = 14
+    rng = np.random.default_rng(width)
+    one, two, three = catalog(width)
+    fams = one + two + three
+    seq = []
+    for _ in range(150):
+        name, gate = fams[int(rng.integers(len(fams)))]
+        seq.append((gate, [int(x) for x in rng.choice(n, size=gate.getQubitsNumber(), replace=False)]))
+    psi = random_state(rng, n)
+    g, o = pair(n, psi, precision)
+    g.apply_circuit(seq, fuse_width=width)
+    for gate, qs in seq:
+        o.apply_gate(gate, *qs)
+    err = np.max(np.abs(g.getRegisterStorage() - o.state()))
+    assert err < tol, err
