@@ -179,6 +179,53 @@ int qcs_ising_energy(qcs_register* r, uint32_t n_edges, const uint32_t* ei,
                      const uint32_t* ej, const double* J, const double* h,
                      double* energy);
 
+/* ---- batched QAOA (caller of the path: IsingModel.h:215-665) ------------- */
+
+/* A batch of weighted Ising instances of uneven size, each evaluated with
+ * QAOAIsingSubalgorithm::Exec's circuit (IsingModel.h:557-567: H on all,
+ * then per layer the cost operator of :491-531 and Rx(2 beta) on all qubits,
+ * :533-554) on a complex64 state, and its exact FP64 energy
+ * sum_s |psi_s|^2 E(s), E(s) = -(sum h z + sum J z z) (:106-119, 203-206),
+ * where the reference samples 100000 shots (:569-578). Instance i has
+ * nqubits[i] qubits and nedges[i] edges (ei, ej, J concatenated; h
+ * concatenated per instance). One persistent launch per stage; the stages of
+ * an evaluation are captured in one hipGraph. */
+typedef struct qcs_qaoa qcs_qaoa;
+int qcs_qaoa_create(uint32_t ninst, const uint32_t* nqubits, const uint32_t* nedges,
+                    const uint32_t* ei, const uint32_t* ej, const double* J,
+                    const double* h, uint32_t p, qcs_qaoa** out);
+int qcs_qaoa_destroy(qcs_qaoa* q);
+/* tile_bits (6..13): instances up to 2^tile_bits amplitudes stay in LDS;
+ * group (4..6): high qubits per register pass; use_graph: replay a hipGraph. */
+int qcs_qaoa_configure(qcs_qaoa* q, uint32_t tile_bits, uint32_t group, int use_graph);
+/* gamma, beta: ninst x p (row per instance); energy: ninst. */
+int qcs_qaoa_evaluate(qcs_qaoa* q, const double* gamma, const double* beta, double* energy);
+int qcs_qaoa_last_ms(qcs_qaoa* q, float* ms);
+/* The final state of one instance as interleaved doubles (2^n of them). */
+int qcs_qaoa_get_state(qcs_qaoa* q, uint32_t inst, double* amps);
+int qcs_qaoa_info(qcs_qaoa* q, uint32_t* stages, uint64_t* amplitudes,
+                  uint32_t* tile_bits, uint32_t* group);
+
+/* ---- MLP surrogate of QAOA energies (no counterpart in the reference) ---- */
+
+/* in -> hidden (x layers, ReLU) -> 1; BF16 weights and activations, FP32
+ * accumulation, FP32 master weights and gradients, Adam, MSE loss. The
+ * parameters are one flat FP32 vector: per layer W (in x out, row-major)
+ * then b (out). */
+typedef struct qcs_mlp qcs_mlp;
+int qcs_mlp_create(uint32_t in_dim, uint32_t hidden, uint32_t layers, uint64_t seed, qcs_mlp** out);
+int qcs_mlp_destroy(qcs_mlp* n);
+uint64_t qcs_mlp_num_params(const qcs_mlp* n);
+int qcs_mlp_get_params(qcs_mlp* n, float* params);
+int qcs_mlp_set_params(qcs_mlp* n, const float* params);
+/* x: batch x in_dim (FP32, rounded to BF16 on upload); y: batch. */
+int qcs_mlp_forward(qcs_mlp* n, const float* x, uint32_t batch, float* y);
+/* Loss (mean squared error against t) and the flat FP32 gradient. */
+int qcs_mlp_gradients(qcs_mlp* n, const float* x, const float* t, uint32_t batch, float* grad, float* loss);
+/* Shuffled mini-batch Adam over a device-resident copy of (X, Y). */
+int qcs_mlp_train(qcs_mlp* n, const float* X, const float* Y, uint64_t rows, uint32_t batch,
+                  uint32_t epochs, float lr, uint64_t seed, float* loss_per_epoch);
+
 /* ---- autotuning (no counterpart in the reference) ------------------------ */
 
 /* Times every launch variant of the streaming gate kernels on this register

@@ -70,6 +70,23 @@ SIGNATURES = {
     "qcs_autotune_variants": (_I, []),
     "qcs_set_tuned_variant": (_I, [_I, _U32, _U32, _I]),
     "qcs_get_tuned_variant": (_I, [_I, _U32, _U32]),
+    "qcs_qaoa_create": (_I, [_U32, _PU32, _PU32, _PU32, _PU32, _PD, _PD, _U32, C.POINTER(_P)]),
+    "qcs_qaoa_destroy": (_I, [_P]),
+    "qcs_qaoa_configure": (_I, [_P, _U32, _U32, _I]),
+    "qcs_qaoa_evaluate": (_I, [_P, _PD, _PD, _PD]),
+    "qcs_qaoa_last_ms": (_I, [_P, C.POINTER(C.c_float)]),
+    "qcs_qaoa_get_state": (_I, [_P, _U32, _PD]),
+    "qcs_qaoa_info": (_I, [_P, _PU32, _PU64, _PU32, _PU32]),
+    "qcs_mlp_create": (_I, [_U32, _U32, _U32, _U64, C.POINTER(_P)]),
+    "qcs_mlp_destroy": (_I, [_P]),
+    "qcs_mlp_num_params": (_U64, [_P]),
+    "qcs_mlp_get_params": (_I, [_P, C.POINTER(C.c_float)]),
+    "qcs_mlp_set_params": (_I, [_P, C.POINTER(C.c_float)]),
+    "qcs_mlp_forward": (_I, [_P, C.POINTER(C.c_float), _U32, C.POINTER(C.c_float)]),
+    "qcs_mlp_gradients": (_I, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float), _U32, C.POINTER(C.c_float),
+                               C.POINTER(C.c_float)]),
+    "qcs_mlp_train": (_I, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float), _U64, _U32, _U32, C.c_float, _U64,
+                           C.POINTER(C.c_float)]),
     "qcs_synchronize": (_I, [_P]),
     "qcs_timer_begin": (_I, [_P]),
     "qcs_timer_end": (_I, [_P, C.POINTER(C.c_float)]),

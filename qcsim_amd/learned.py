@@ -1,0 +1,173 @@
+"""Learned angle proposals for QAOA on Ising instances.
+
+An MLP (qcs_mlp_*: BF16 weights and activations, FP32 accumulation, trained
+by the project's HIP forward/backward/Adam kernels) maps instance statistics
+plus the 2p normalised angles to the energy, normalised by the instance's
+scale S = sum|J| + sum|h|. It is trained on (instance, angles, exact energy)
+tuples that the batched simulator (QaoaBatch) generates. For a new instance
+it scores many random angle sets, and the best `top` of them are re-scored
+exactly on the statevector; the plain optimiser (qaoa.plain_optimize) remains
+the fallback, warm-started from the best proposal.
+
+    python -m qcsim_amd.learned --train-rows 1000000 --sizes 16,18,20 --test-n 20
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import math
+import time
+
+import numpy as np
+
+from ._lib import check, lib
+from .qaoa import IsingInstance, QaoaBatch, plain_optimize, random_ising
+
+_PF = C.POINTER(C.c_float)
+N_STATS = 9
+
+
+def instance_stats(inst: IsingInstance) -> np.ndarray:
+    """Graph / J / h statistics (degree, coupling and field moments)."""
+    J = np.asarray(inst.J, dtype=np.float64)
+    h = np.asarray(inst.h, dtype=np.float64)
+    deg = np.zeros(inst.n)
+    for i, j in inst.edges:
+        deg[i] += 1
+        deg[j] += 1
+    scale = np.sum(np.abs(J)) + np.sum(np.abs(h))
+    return np.array(
+        [inst.n / 24.0, deg.mean() / 3.0, J.mean(), J.std(), np.abs(J).mean(), h.mean(), h.std(), np.abs(h).mean(),
+         scale / inst.n],
+        dtype=np.float64,
+    )
+
+
+def energy_scale(inst: IsingInstance) -> float:
+    return float(np.sum(np.abs(inst.J)) + np.sum(np.abs(inst.h)))
+
+
+def features(stats: np.ndarray, gammas: np.ndarray, betas: np.ndarray) -> np.ndarray:
+    """Rows of [stats, gamma/pi, beta/pi] (angles are taken in [0, pi))."""
+    G = np.atleast_2d(gammas) / math.pi
+    B = np.atleast_2d(betas) / math.pi
+    S = np.broadcast_to(stats, (G.shape[0], stats.shape[0]))
+    return np.concatenate([S, G, B], axis=1).astype(np.float32)
+
+
+class Surrogate:
+    """The MLP on the GPU (qcs_mlp_*)."""
+
+    def __init__(self, in_dim: int, hidden: int = 256, layers: int = 3, seed: int = 1):
+        h = C.c_void_p()
+        check(lib().qcs_mlp_create(in_dim, hidden, layers, seed, C.byref(h)))
+        self._h = h
+        self.in_dim = in_dim
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().qcs_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def train(self, X: np.ndarray, Y: np.ndarray, batch=4096, epochs=20, lr=1e-3, seed=3) -> np.ndarray:
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.ascontiguousarray(Y, dtype=np.float32)
+        losses = np.zeros(epochs, dtype=np.float32)
+        check(lib().qcs_mlp_train(self._h, X.ctypes.data_as(_PF), Y.ctypes.data_as(_PF), X.shape[0], batch, epochs, lr,
+                                  seed, losses.ctypes.data_as(_PF)))
+        return losses
+
+    def predict(self, X: np.ndarray) -> np.ndarray:
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.zeros(X.shape[0], dtype=np.float32)
+        check(lib().qcs_mlp_forward(self._h, X.ctypes.data_as(_PF), X.shape[0], y.ctypes.data_as(_PF)))
+        return y
+
+
+def generate_dataset(sizes, n_instances: int, rows: int, p: int, rng: np.random.Generator, chunk_instances=256):
+    """(features, normalised energies) from random instances and angles, all
+    energies exact (QaoaBatch). Returns X, Y, seconds of GPU evaluation."""
+    per_inst = max(1, rows // n_instances)
+    Xs, Ys = [], []
+    t_eval = 0.0
+    for c0 in range(0, n_instances, chunk_instances):
+        insts = [random_ising(int(rng.choice(sizes)), rng) for _ in range(min(chunk_instances, n_instances - c0))]
+        stats = [instance_stats(I) for I in insts]
+        scales = np.array([energy_scale(I) for I in insts])
+        batch = QaoaBatch(insts, p)
+        for _ in range(per_inst):
+            G = rng.uniform(0, math.pi, (len(insts), p))
+            B = rng.uniform(0, math.pi, (len(insts), p))
+            t0 = time.perf_counter()
+            E = batch.evaluate(G, B)
+            t_eval += time.perf_counter() - t0
+            for k in range(len(insts)):
+                Xs.append(features(stats[k], G[k], B[k]))
+            Ys.append(E / scales)
+        batch.close()
+    X = np.concatenate(Xs, axis=0)
+    Y = np.concatenate(Ys).astype(np.float32)
+    return X, Y, t_eval
+
+
+def propose(model: Surrogate, inst: IsingInstance, p: int, rng: np.random.Generator, candidates=65536, top=32):
+    """Top-`top` angle sets by predicted energy among random candidates."""
+    G = rng.uniform(0, math.pi, (candidates, p))
+    B = rng.uniform(0, math.pi, (candidates, p))
+    pred = model.predict(features(instance_stats(inst), G, B))
+    idx = np.argsort(pred)[:top]
+    return G[idx], B[idx], pred[idx] * energy_scale(inst)
+
+
+def learned_search(model: Surrogate, inst: IsingInstance, p: int, rng: np.random.Generator, top=32, candidates=65536):
+    """Propose, re-score all proposals exactly; returns (best E, angles, evals)."""
+    G, B, _ = propose(model, inst, p, rng, candidates, top)
+    batch = QaoaBatch([inst] * top, p)
+    E = batch.evaluate(G, B)
+    batch.close()
+    k = int(np.argmin(E))
+    return float(E[k]), (G[k], B[k]), top
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--train-rows", type=int, default=1_000_000)
+    ap.add_argument("--train-instances", type=int, default=2048)
+    ap.add_argument("--sizes", default="16,18,20")
+    ap.add_argument("--p", type=int, default=4)
+    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--test-n", type=int, default=20)
+    ap.add_argument("--test-instances", type=int, default=4)
+    ap.add_argument("--seed", type=int, default=2024)
+    args = ap.parse_args()
+    rng = np.random.default_rng(args.seed)
+    sizes = [int(s) for s in args.sizes.split(",")]
+    t0 = time.perf_counter()
+    X, Y, t_eval = generate_dataset(sizes, args.train_instances, args.train_rows, args.p, rng)
+    t_gen = time.perf_counter() - t0
+    model = Surrogate(X.shape[1])
+    t0 = time.perf_counter()
+    losses = model.train(X, Y, epochs=args.epochs)
+    t_train = time.perf_counter() - t0
+    results = []
+    for _ in range(args.test_instances):
+        inst = random_ising(args.test_n, rng)
+        factory = lambda insts, p: QaoaBatch(insts, p)  # noqa: E731
+        e_plain, _, ev_plain, _ = plain_optimize(factory, inst, [1.0] * args.p, [1.0] * args.p)
+        e_learn, (g, b), ev_learn = learned_search(model, inst, args.p, rng)
+        results.append({"plain_E": e_plain, "plain_evals": ev_plain, "learned_E": e_learn, "learned_evals": ev_learn,
+                        "within_1e-3": e_learn <= e_plain + 1e-3, "eval_ratio": ev_plain / ev_learn})
+    print(json.dumps({"rows": int(X.shape[0]), "gen_s": round(t_gen, 1), "gpu_eval_s": round(t_eval, 1),
+                      "train_s": round(t_train, 1), "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
+                      "results": results}))
+
+
+if __name__ == "__main__":
+    main()
