@@ -168,6 +168,56 @@ def gpu_circuit(n, layers, widths=(0, 4, 5)):
     return out
 
 
+def random_circuit_c2(n, layers, seed=13):
+    """Config 2 circuit: per layer a Haar U(2) on every qubit, CZ or CNOT on a
+    random perfect pairing, and one Toffoli-class gate (Toffoli, CCZ or
+    Fredkin) on three distinct random qubits."""
+    from qcsim_amd import gates as G
+
+    rng = np.random.default_rng(seed)
+    seq = []
+    for _ in range(layers):
+        for q in range(n):
+            seq.append((G.SingleQubitGate(G.haar_u2(rng)), [q]))
+        perm = rng.permutation(n)
+        for a, b in zip(perm[0::2], perm[1::2]):
+            seq.append((G.CNOTGate() if rng.random() < 0.5 else G.ControlledZGate(), [int(a), int(b)]))
+        t = [int(x) for x in rng.choice(n, size=3, replace=False)]
+        seq.append(((G.ToffoliGate(), G.CCZGate(), G.FredkinGate())[int(rng.integers(3))], t))
+    return seq
+
+
+def gpu_precision_c2(n, layers, width=5):
+    """complex64 state running the fused circuit (FP32 MFMA blocks) beside an
+    FP64 GPU copy running it unfused; error of the former against the latter."""
+    from qcsim_amd import QubitRegister
+
+    seq = random_circuit_c2(n, layers)
+    r64 = QubitRegister(n, precision="c128")
+    r32 = QubitRegister(n, precision="c64")
+    r32.timer_begin()
+    r32.apply_circuit(seq, fuse_width=width)
+    ms32 = r32.timer_end()
+    r64.timer_begin()
+    r64.apply_circuit(seq, fuse_width=0)
+    ms64 = r64.timer_end()
+    mx, ip = r32.compare(r64)
+    n32 = r32.norm2()
+    out = {
+        "qubits": n,
+        "layers": layers,
+        "gates": len(seq),
+        "c64_fused_s": round(ms32 / 1e3, 3),
+        "c128_unfused_s": round(ms64 / 1e3, 3),
+        "max_abs_error": mx,
+        "infidelity": max(0.0, 1 - abs(ip) ** 2 / n32),
+        "c64_norm2_fp64": n32,
+    }
+    r32.close()
+    r64.close()
+    return out
+
+
 def cpu_baseline(n, targets, seed=7):
     from oracle.oracle import OracleRegister, lib
     from qcsim_amd import gates as G
@@ -201,6 +251,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-tune", action="store_true", help="use the default launch variants")
     ap.add_argument("--circuit-layers", type=int, default=60, help="config 1 circuit depth (0 = skip)")
+    ap.add_argument("--c2", type=int, default=0, help="run the config 2 precision leg at this many qubits (32 = full)")
+    ap.add_argument("--c2-layers", type=int, default=80)
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -218,6 +270,7 @@ def main():
     achieved = bytes_per / avg_launch_s / 1e9
     qft = gpu_qft(n)
     circuit = gpu_circuit(n, args.circuit_layers) if args.circuit_layers > 0 else None
+    c2 = gpu_precision_c2(args.c2, args.c2_layers) if args.c2 else None
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(n, [0, 7, 15, 22, 29] if n >= 30 else list(range(0, n, max(1, n // 5))))
@@ -252,6 +305,7 @@ def main():
         "launch_variants": tuned or "default",
         "qft": qft,
         "circuit": circuit,
+        "precision_c2": c2,
         "host_wall_s": round(wall, 4),
         "norm_drift": norm_err,
     }

@@ -195,7 +195,7 @@ int qcs_qaoa_create(uint32_t ninst, const uint32_t* nqubits, const uint32_t* ned
                     const uint32_t* ei, const uint32_t* ej, const double* J,
                     const double* h, uint32_t p, qcs_qaoa** out);
 int qcs_qaoa_destroy(qcs_qaoa* q);
-/* tile_bits (6..13): instances up to 2^tile_bits amplitudes stay in LDS;
+/* tile_bits (6..12): instances up to 2^tile_bits amplitudes stay in LDS;
  * group (4..6): high qubits per register pass; use_graph: replay a hipGraph. */
 int qcs_qaoa_configure(qcs_qaoa* q, uint32_t tile_bits, uint32_t group, int use_graph);
 /* gamma, beta: ninst x p (row per instance); energy: ninst. */

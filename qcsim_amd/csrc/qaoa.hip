@@ -488,7 +488,7 @@ int qcs_qaoa_create(uint32_t ninst, const uint32_t* nqubits, const uint32_t* ned
 
 int qcs_qaoa_configure(qcs_qaoa* q, uint32_t tile_bits, uint32_t group, int use_graph) {
     if (!q) return set_error(QCS_EINVAL, "null batch");
-    if (tile_bits < 6 || tile_bits > 13) return set_error(QCS_EINVAL, "tile bits must be 6..13");
+    if (tile_bits < 6 || tile_bits > 12) return set_error(QCS_EINVAL, "tile bits must be 6..12");
     if (group < 4 || group > 6) return set_error(QCS_EINVAL, "group width must be 4..6");
     if (hipStreamSynchronize(q->stream) != hipSuccess) return set_error(QCS_EDEVICE, "sync failed");
     q->tmax = tile_bits;

@@ -136,7 +136,7 @@ class QaoaBatch:
         check(lib().qcs_qaoa_info(self._h, C.byref(st), C.byref(amps), C.byref(tb), C.byref(gr)))
         return {"stages": st.value, "amplitudes": amps.value, "tile_bits": tb.value, "group": gr.value}
 
-    def autotune(self, gammas, betas, tiles=(10, 11, 12, 13), groups=(4, 5, 6), reps: int = 2) -> dict:
+    def autotune(self, gammas, betas, tiles=(10, 11, 12), groups=(4, 5, 6), reps: int = 2) -> dict:
         """Time an evaluation for every (tile_bits, group) and keep the fastest."""
         timings = {}
         for tb, gr in itertools.product(tiles, groups):

@@ -1,8 +1,10 @@
 """The learned proposer end to end at test scale: the batched simulator
 generates (instance, angles, exact energy) tuples, the MLP is trained by the
-HIP kernels, and on new instances the top-32 proposals, re-scored exactly,
+HIP kernels, and on new instances the best proposals, re-scored exactly,
 reach the plain optimiser's best energy (the reference's ADAM-W loop on exact
-energies) within 1e-3 with at least 5x fewer exact evaluations."""
+energies, which stops after 7-9 iterations = 119-153 evaluations on these
+12-qubit instances) within 1e-3: with the top 16 proposals at >= 5x fewer
+exact evaluations, with the top 32 at >= 3.5x fewer."""
 import numpy as np
 import pytest
 
@@ -20,14 +22,16 @@ def test_learned_proposals_beat_plain_evaluation_count():
     losses = model.train(X, Y, batch=2048, epochs=25, lr=1e-3)
     assert losses[-1] < 0.5 * losses[0]
     factory = lambda insts, pp: QaoaBatch(insts, pp)  # noqa: E731
-    wins, rows = 0, []
+    wins16, wins32, rows = 0, 0, []
     for k in range(4):
         inst = random_ising(12, rng)
-        e_plain, _, ev_plain, _ = plain_optimize(factory, inst, [1.0] * p, [1.0] * p)
-        e_learn, _, ev_learn = learned_search(model, inst, p, rng, top=32)
-        rows.append((e_plain, ev_plain, e_learn, ev_learn))
-        if e_learn <= e_plain + 1e-3 and ev_plain >= 5 * ev_learn:
-            wins += 1
-    print(rows)
-    assert wins >= 3, rows
+        e_plain, _, ev_plain, _ = plain_optimize(factory, inst, [1.0] * p, [1.0] * p, max_iter=3000)
+        e16, _, ev16 = learned_search(model, inst, p, rng, top=16)
+        e32, _, ev32 = learned_search(model, inst, p, rng, top=32)
+        rows.append((round(e_plain, 4), ev_plain, round(e16, 4), ev16, round(e32, 4), ev32))
+        wins16 += e16 <= e_plain + 1e-3 and ev_plain >= 5 * ev16
+        wins32 += e32 <= e_plain + 1e-3 and ev_plain >= 3.5 * ev32
+    print("plain E, evals | learned top-16 E, evals | top-32 E, evals:", rows)
+    assert wins16 >= 3, rows
+    assert wins32 >= 3, rows
     model.close()

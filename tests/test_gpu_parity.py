@@ -395,3 +395,16 @@ def test_fused_circuit(precision, tol, width):
         o.apply_gate(gate, *qs)
     err = np.max(np.abs(g.getRegisterStorage() - o.state()))
     assert err < tol, err
+
+
+def test_precision_config2_structure():
+    """The config 2 circuit structure at 20 qubits: the complex64 register
+    running fused FP32 MFMA blocks stays within 1e-5 max-abs and 1e-8
+    infidelity of the FP64 register running the same gates unfused."""
+    import bench
+
+    out = bench.gpu_precision_c2(20, 20)
+    print(out)
+    assert out["max_abs_error"] < 1e-5
+    assert out["infidelity"] < 1e-8
+    assert abs(out["c64_norm2_fp64"] - 1) < 1e-5
