@@ -57,6 +57,20 @@ __global__ void k_f2d(const float2* __restrict__ in, double2* __restrict__ out, 
     if (i < n) out[i] = make_double2(double(in[i].x), double(in[i].y));
 }
 
+
+int stage_upload(qcs_register* r, const void* src, size_t bytes, const void** dev) {
+    if (bytes > r->mat_bytes) return set_error(QCS_EINVAL, "parameter block too large");
+    const uint32_t slot = r->mat_next++ % qcs_register::kMatSlots;
+    if (hipEventSynchronize(r->mat_ev[slot]) != hipSuccess) return set_error(QCS_EDEVICE, "staging wait failed");
+    char* h = static_cast<char*>(r->hmat) + slot * r->mat_bytes;
+    char* d = static_cast<char*>(r->mat) + slot * r->mat_bytes;
+    std::memcpy(h, src, bytes);
+    if (hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, r->stream) != hipSuccess ||
+        hipEventRecord(r->mat_ev[slot], r->stream) != hipSuccess)
+        return set_error(QCS_EDEVICE, "parameter upload failed");
+    *dev = d;
+    return QCS_OK;
+}
 }  // namespace qcs
 
 using namespace qcs;
@@ -99,7 +113,10 @@ int qcs_create(uint32_t nqubits, int precision, qcs_register** out) {
         if (r->chunk) hipFree(r->chunk);
         if (r->dout) hipFree(r->dout);
         if (r->mat) hipFree(r->mat);
+        if (r->hmat) hipHostFree(r->hmat);
         if (r->hpin) hipHostFree(r->hpin);
+        for (hipEvent_t ev : r->mat_ev)
+            if (ev) hipEventDestroy(ev);
         if (r->stream) hipStreamDestroy(r->stream);
         delete r;
         return set_error(code, "%s", what);
@@ -112,15 +129,20 @@ int qcs_create(uint32_t nqubits, int precision, qcs_register** out) {
     if (hipMalloc(reinterpret_cast<void**>(&r->red), sizeof(double) * (kRedBlocks * 3 + 64)) != hipSuccess ||
         hipMalloc(reinterpret_cast<void**>(&r->chunk), sizeof(double) * nchunks) != hipSuccess ||
         hipMalloc(reinterpret_cast<void**>(&r->dout), sizeof(uint64_t) * 64) != hipSuccess ||
-        hipMalloc(&r->mat, r->mat_bytes) != hipSuccess) {
+        hipMalloc(&r->mat, r->mat_bytes * qcs_register::kMatSlots) != hipSuccess) {
         hipGetLastError();
         return fail(QCS_ENOMEM, "hipMalloc of scratch failed");
     }
-    if (hipHostMalloc(reinterpret_cast<void**>(&r->hpin), sizeof(double) * 64) != hipSuccess)
+    if (hipHostMalloc(reinterpret_cast<void**>(&r->hpin), sizeof(double) * 64) != hipSuccess ||
+        hipHostMalloc(&r->hmat, r->mat_bytes * qcs_register::kMatSlots) != hipSuccess)
         return fail(QCS_ENOMEM, "hipHostMalloc failed");
     if (hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&r->ev0) != hipSuccess || hipEventCreate(&r->ev1) != hipSuccess)
         return fail(QCS_EDEVICE, "stream/event creation failed");
+    for (hipEvent_t& ev : r->mat_ev)
+        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess ||
+            hipEventRecord(ev, r->stream) != hipSuccess)
+            return fail(QCS_EDEVICE, "staging event creation failed");
     int e = kfill(r, 0, 0);
     if (!e) e = kset_one(r, 0, 1, 0);
     if (!e && hipStreamSynchronize(r->stream) != hipSuccess) e = set_error(QCS_EDEVICE, "init sync failed");
@@ -140,7 +162,10 @@ int qcs_destroy(qcs_register* r) {
     hipFree(r->chunk);
     hipFree(r->dout);
     hipFree(r->mat);
+    hipHostFree(r->hmat);
     hipHostFree(r->hpin);
+    for (hipEvent_t ev : r->mat_ev)
+        if (ev) hipEventDestroy(ev);
     if (r->ev0) hipEventDestroy(r->ev0);
     if (r->ev1) hipEventDestroy(r->ev1);
     if (r->stream) hipStreamDestroy(r->stream);
@@ -156,9 +181,12 @@ int qcs_copy_state(qcs_register* dst, const qcs_register* src) {
     QCS_REQUIRE(dst && src, "null register");
     QCS_REQUIRE(dst->A == src->A && dst->prec == src->prec, "registers differ in size or precision");
     if (dst == src) return QCS_OK;
+    // The copy runs on dst's stream after src's pending work; it completes
+    // before returning, so either register may be modified right after.
     if (hipStreamSynchronize(src->stream) != hipSuccess) return set_error(QCS_EDEVICE, "sync failed");
     if (hipMemcpyAsync(dst->psi, src->psi, src->A * src->amp_bytes(), hipMemcpyDeviceToDevice, dst->stream) !=
-        hipSuccess)
+            hipSuccess ||
+        hipStreamSynchronize(dst->stream) != hipSuccess)
         return set_error(QCS_EDEVICE, "device copy failed");
     return QCS_OK;
 }
@@ -391,9 +419,13 @@ int qcs_apply_diagonal(qcs_register* r, const double* diag, uint64_t n) {
     double* d = nullptr;
     if (hipMallocAsync(reinterpret_cast<void**>(&d), n * 16, r->stream) != hipSuccess)
         return set_error(QCS_ENOMEM, "diagonal buffer");
-    hipMemcpyAsync(d, diag, n * 16, hipMemcpyHostToDevice, r->stream);
-    const int e = kdiag_full(r, d);
+    // `diag` is pageable caller memory: wait for the kernel before returning
+    // so the caller may release it.
+    int e = hipMemcpyAsync(d, diag, n * 16, hipMemcpyHostToDevice, r->stream) == hipSuccess
+                ? kdiag_full(r, d)
+                : set_error(QCS_EDEVICE, "diagonal upload failed");
     hipFreeAsync(d, r->stream);
+    if (hipStreamSynchronize(r->stream) != hipSuccess && !e) e = set_error(QCS_EDEVICE, "diagonal sync failed");
     return e;
 }
 

@@ -109,7 +109,14 @@ struct qcs_register {
     double* red = nullptr;      // reduction partials (kRedBlocks * 4 doubles)
     double* chunk = nullptr;    // per-chunk CDF (A / kChunk doubles, >= 1)
     uint64_t* dout = nullptr;   // search results / small device outputs
-    void* mat = nullptr;        // dense matrix staging (device)
+    // Parameter staging ring for dense matrices: kMatSlots slots of
+    // mat_bytes each, pinned on the host and mirrored on the device. A host
+    // slot is refilled only after the event of its previous copy completed.
+    static constexpr int kMatSlots = 8;
+    void* mat = nullptr;        // device slots
+    void* hmat = nullptr;       // pinned host slots
+    hipEvent_t mat_ev[kMatSlots] = {};
+    uint32_t mat_next = 0;
     size_t mat_bytes = 0;
     double* hpin = nullptr;     // pinned host scalars
     hipStream_t stream = nullptr;
@@ -129,6 +136,10 @@ constexpr int kScanThreads = 256;
 
 int set_error(int code, const char* fmt, ...);
 int check_launch(qcs_register* r, const char* what);
+// Copies `bytes` (<= mat_bytes) from host memory into the next staging slot
+// and enqueues its upload on the register's stream; *dev receives the device
+// slot. `src` may be released as soon as the call returns.
+int stage_upload(qcs_register* r, const void* src, size_t bytes, const void** dev);
 
 // Launch variant of the streaming gate kernels: orbits per thread (1, 2, 4),
 // nontemporal access and XCD-aware block order. Chosen per (precision,

@@ -171,8 +171,8 @@ static int kdense_mfma(qcs_register* r, const uint32_t* qubits, const double* ma
             R[size_t(D + i) * RD + j] = im;
             R[size_t(D + i) * RD + D + j] = re;
         }
-    if (hipMemcpyAsync(r->mat, R.data(), sizeof(T) * R.size(), hipMemcpyHostToDevice, r->stream) != hipSuccess)
-        return set_error(QCS_EDEVICE, "dense matrix upload failed");
+    const void* dmat = nullptr;
+    if (const int e = stage_upload(r, R.data(), sizeof(T) * R.size(), &dmat)) return e;
     TileGeom tg{};
     uint64_t gmask = 0;
     for (int i = 0; i < K; ++i) gmask |= 1ull << qubits[i];
@@ -195,7 +195,7 @@ static int kdense_mfma(qcs_register* r, const uint32_t* qubits, const double* ma
     const uint64_t ntiles = r->A >> (K + 6);
     const unsigned grid = unsigned(ntiles < 4096 ? ntiles : 4096);
     hipLaunchKernelGGL((k_dense_mfma<T, K>), dim3(grid), dim3(256), 0, r->stream, static_cast<cv<T>*>(r->psi), ntiles,
-                       tg, static_cast<const T*>(r->mat));
+                       tg, static_cast<const T*>(dmat));
     return check_launch(r, "k_dense_mfma");
 }
 
@@ -203,15 +203,14 @@ template <typename T, int K>
 static int kdense_big(qcs_register* r, const uint32_t* qubits, const double* mat_host) {
     if (r->n >= uint32_t(K + 6) && !r->dense_no_mfma) return kdense_mfma<T, K>(r, qubits, mat_host);
     constexpr int D = 1 << K;
-    // matrix converted to T, split re[] then im[], uploaded from pageable
-    // memory (the copy has consumed the host buffer when the call returns).
+    // matrix converted to T, split re[] then im[], through the staging ring
     std::vector<T> hm(2 * D * D);
     for (int i = 0; i < D * D; ++i) {
         hm[i] = T(mat_host[2 * i]);
         hm[D * D + i] = T(mat_host[2 * i + 1]);
     }
-    if (hipMemcpyAsync(r->mat, hm.data(), sizeof(T) * hm.size(), hipMemcpyHostToDevice, r->stream) != hipSuccess)
-        return set_error(QCS_EDEVICE, "dense matrix upload failed");
+    const void* dmat = nullptr;
+    if (const int e = stage_upload(r, hm.data(), sizeof(T) * hm.size(), &dmat)) return e;
     Ins ins{};
     uint32_t tmp[K];
     QBits<K> qb;
@@ -225,7 +224,7 @@ static int kdense_big(qcs_register* r, const uint32_t* qubits, const double* mat
     const uint64_t norb = r->A >> K;
     const unsigned g = unsigned((norb + 255) / 256);
     hipLaunchKernelGGL((k_dense<T, K>), dim3(g), dim3(256), 0, r->stream, static_cast<cv<T>*>(r->psi), norb, ins,
-                       qb, static_cast<const T*>(r->mat));
+                       qb, static_cast<const T*>(dmat));
     return check_launch(r, "k_dense");
 }
 

@@ -186,6 +186,9 @@ struct qcs_mlp {
     uint64_t nrows = 0;
     hipStream_t stream = nullptr;
     uint64_t step = 0;
+    // host staging of the bf16 inputs; refilled only after the stream has
+    // drained, so its previous upload has completed
+    std::vector<bf16> hx;
 };
 
 static int mlp_reserve(qcs_mlp* n, int B) {
@@ -353,7 +356,9 @@ int qcs_mlp_set_params(qcs_mlp* n, const float* p) {
 }
 
 static int upload_inputs(qcs_mlp* n, const float* x, int B) {
-    std::vector<bf16> xb(size_t(B) * n->in);
+    if (hipStreamSynchronize(n->stream) != hipSuccess) return set_error(QCS_EDEVICE, "sync failed");
+    std::vector<bf16>& xb = n->hx;
+    xb.resize(size_t(B) * n->in);
     for (size_t i = 0; i < xb.size(); ++i) xb[i] = bf16_from(x[i]);
     return hipMemcpyAsync(n->x0, xb.data(), sizeof(bf16) * xb.size(), hipMemcpyHostToDevice, n->stream) == hipSuccess
                ? QCS_OK

@@ -15,12 +15,13 @@
 //
 // Work decomposition for an n-qubit instance with group width G:
 //   n <= Tmax: one SMALL unit runs every layer and the energy in LDS;
-//   else tile bits T = n - G * ceil((n - Tmax) / G) and per layer
+//   else, with tile bits T = Tmax, per layer
 //     stage A: 2^(n-T) units, each a 2^T tile: (layer 0: |+> generated in
 //              place) cost diagonal, Rx on qubits 0..T-1 in LDS;
-//     stage B_j (j = 0 .. (n-T)/G - 1): Rx on qubits T+jG .. T+jG+G-1 with
-//              each thread's 2^G-amplitude orbit in registers; the last
-//              stage of the last layer also accumulates the energy.
+//     stage B_j (j = 0 .. ceil((n-T)/G) - 1): Rx on qubits T+jG .. (the
+//              last group may be narrower) with each thread's orbit in
+//              registers; the last stage of the last layer also
+//              accumulates the energy.
 // A stage's kernel pulls units from an atomic ticket until none are left.
 #include <algorithm>
 #include <cmath>
@@ -45,7 +46,7 @@ enum : uint32_t { U_SMALL = 0, U_A = 1, U_B = 2 };
 
 struct QUnit {
     uint32_t inst;
-    uint32_t kind;   // U_* | (energy << 8)
+    uint32_t kind;   // U_* | (energy << 8) | (group width << 16)
     uint32_t idx;    // tile (A) or 256-orbit chunk (B)
     uint32_t layer;
     uint32_t qlo;    // first qubit of the B group
@@ -111,6 +112,42 @@ __device__ void lds_rx(float2* t, uint32_t tb, float c, float s) {
         }
         __syncthreads();
     }
+}
+
+// Rx(2 beta) on qubits qlo .. qlo+g-1, each thread one 2^g-amplitude orbit
+// in registers; returns this thread's share of the energy when asked.
+template <int g>
+__device__ double stage_b(float2* __restrict__ psi, const QUnit& U, const QInst& I, const double* ga, bool want_e,
+                          const double* sh, const double* sJ, const uint8_t* si, const uint8_t* sj) {
+    constexpr int D = 1 << g;
+    const uint64_t o = uint64_t(U.idx) * 256 + threadIdx.x;
+    const uint64_t norb = (1ull << I.n) >> g;
+    double acc = 0;
+    if (o >= norb) return 0;
+    const uint64_t base = insert_run(o, U.qlo, g);
+    float2 x[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) x[c] = ldamp<float, true>(psi + (base | (uint64_t(c) << U.qlo)));
+    double sb, cb;
+    sincos(ga[2 * U.layer + 1], &sb, &cb);
+    const float fc = float(cb), fs = float(sb);
+#pragma unroll
+    for (int q = 0; q < g; ++q)
+#pragma unroll
+        for (int c = 0; c < D; ++c)
+            if (!((c >> q) & 1)) rx_pair(x[c], x[c | (1 << q)], fc, fs);
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+        const uint64_t s = base | (uint64_t(c) << U.qlo);
+        if (want_e) {
+            double cc, e;
+            ising_terms(s, I.n, I.ne, sh, sJ, si, sj, cc, e);
+            const double a = x[c].x, b = x[c].y;
+            acc += (a * a + b * b) * e;
+        }
+        stamp<float, true>(psi + s, x[c]);
+    }
+    return acc;
 }
 
 template <int G>
@@ -182,34 +219,15 @@ __global__ __launch_bounds__(256) void k_qaoa_stage(float2* __restrict__ arena, 
             lds_rx(tile, tb, float(cb), float(sb));
             for (uint32_t i = threadIdx.x; i < C; i += 256) stamp<float, true>(psi + base + i, tile[i]);
         } else {
-            constexpr int D = 1 << G;
-            const uint64_t o = uint64_t(U.idx) * 256 + threadIdx.x;
-            const uint64_t norb = (1ull << I.n) >> G;
+            const uint32_t g = (U.kind >> 16) & 0xff;
             double acc = 0;
-            if (o < norb) {
-                const uint64_t base = insert_run(o, U.qlo, G);
-                float2 x[D];
-#pragma unroll
-                for (int c = 0; c < D; ++c) x[c] = ldamp<float, true>(psi + (base | (uint64_t(c) << U.qlo)));
-                double sb, cb;
-                sincos(ga[2 * U.layer + 1], &sb, &cb);
-                const float fc = float(cb), fs = float(sb);
-#pragma unroll
-                for (int q = 0; q < G; ++q)
-#pragma unroll
-                    for (int c = 0; c < D; ++c)
-                        if (!((c >> q) & 1)) rx_pair(x[c], x[c | (1 << q)], fc, fs);
-#pragma unroll
-                for (int c = 0; c < D; ++c) {
-                    const uint64_t s = base | (uint64_t(c) << U.qlo);
-                    if (want_e) {
-                        double cc, e;
-                        ising_terms(s, I.n, I.ne, sh, sJ, si, sj, cc, e);
-                        const double a = x[c].x, b = x[c].y;
-                        acc += (a * a + b * b) * e;
-                    }
-                    stamp<float, true>(psi + s, x[c]);
-                }
+            switch (g) {
+                case 1: acc = stage_b<1>(psi, U, I, ga, want_e, sh, sJ, si, sj); break;
+                case 2: acc = stage_b<2>(psi, U, I, ga, want_e, sh, sJ, si, sj); break;
+                case 3: acc = stage_b<3>(psi, U, I, ga, want_e, sh, sJ, si, sj); break;
+                case 4: acc = stage_b<G >= 4 ? 4 : G>(psi, U, I, ga, want_e, sh, sJ, si, sj); break;
+                case 5: acc = stage_b<G >= 5 ? 5 : G>(psi, U, I, ga, want_e, sh, sJ, si, sj); break;
+                default: acc = stage_b<G>(psi, U, I, ga, want_e, sh, sJ, si, sj); break;
             }
             if (want_e) {
                 const double tot = block_sum(acc, red);
@@ -337,22 +355,26 @@ static int build_plan(qcs_qaoa* q) {
             continue;
         }
         const uint32_t ng = (n - Tm + G - 1) / G;
-        const uint32_t tb = n - G * ng;
+        const uint32_t tb = Tm;
         I.tbits = tb;
         large.push_back(i);
         const uint32_t ntiles = 1u << (n - tb);
-        const uint32_t nchunks = uint32_t((((1ull << n) >> G) + 255) / 256);
+        const uint32_t glast = (n - Tm) - G * (ng - 1);  // width of the last group
+        const uint32_t nchunks_last = uint32_t((((1ull << n) >> glast) + 255) / 256);
         I.pslot0 = pslot;
-        I.npslots = nchunks;
-        pslot += nchunks;
+        I.npslots = nchunks_last;
+        pslot += nchunks_last;
         for (uint32_t l = 0; l < q->p; ++l) {
             auto& A = stage(size_t(l) * per_layer);
             for (uint32_t t = 0; t < ntiles; ++t) A.push_back(QUnit{i, U_A, t, l, 0, 0});
-            for (uint32_t g = 0; g < ng; ++g) {
-                const bool last = (l == q->p - 1) && (g == ng - 1);
-                auto& B = stage(size_t(l) * per_layer + 1 + g);
+            for (uint32_t gi = 0; gi < ng; ++gi) {
+                const uint32_t gw = gi == ng - 1 ? glast : G;
+                const uint32_t nchunks = uint32_t((((1ull << n) >> gw) + 255) / 256);
+                const bool last = (l == q->p - 1) && (gi == ng - 1);
+                auto& B = stage(size_t(l) * per_layer + 1 + gi);
                 for (uint32_t c = 0; c < nchunks; ++c)
-                    B.push_back(QUnit{i, U_B | (last ? 0x100u : 0u), c, l, tb + g * G, last ? I.pslot0 + c : 0});
+                    B.push_back(QUnit{i, U_B | (last ? 0x100u : 0u) | (gw << 16), c, l, tb + gi * G,
+                                      last ? I.pslot0 + c : 0});
             }
         }
     }

@@ -17,6 +17,7 @@ import argparse
 import ctypes as C
 import json
 import math
+import sys
 import time
 
 import numpy as np
@@ -91,9 +92,11 @@ class Surrogate:
         return y
 
 
-def generate_dataset(sizes, n_instances: int, rows: int, p: int, rng: np.random.Generator, chunk_instances=256):
+def generate_dataset(sizes, n_instances: int, rows: int, p: int, rng: np.random.Generator, chunk_instances=256,
+                     progress=None):
     """(features, normalised energies) from random instances and angles, all
-    energies exact (QaoaBatch). Returns X, Y, seconds of GPU evaluation."""
+    energies exact (QaoaBatch). Returns X, Y, seconds of GPU evaluation.
+    `progress(text)` (optional) is called after each chunk of instances."""
     per_inst = max(1, rows // n_instances)
     Xs, Ys = [], []
     t_eval = 0.0
@@ -108,10 +111,12 @@ def generate_dataset(sizes, n_instances: int, rows: int, p: int, rng: np.random.
             t0 = time.perf_counter()
             E = batch.evaluate(G, B)
             t_eval += time.perf_counter() - t0
-            for k in range(len(insts)):
-                Xs.append(features(stats[k], G[k], B[k]))
+            Xs.append(np.concatenate([np.stack(stats), G / math.pi, B / math.pi], axis=1).astype(np.float32))
             Ys.append(E / scales)
         batch.close()
+        if progress:
+            progress(f"dataset: {c0 + len(insts)}/{n_instances} instances, {sum(len(y) for y in Ys)} rows, "
+                     f"gpu eval {t_eval:.1f} s")
     X = np.concatenate(Xs, axis=0)
     Y = np.concatenate(Ys).astype(np.float32)
     return X, Y, t_eval
@@ -149,21 +154,27 @@ def main():
     args = ap.parse_args()
     rng = np.random.default_rng(args.seed)
     sizes = [int(s) for s in args.sizes.split(",")]
+
+    def progress(text):
+        print(text, file=sys.stderr, flush=True)
+
     t0 = time.perf_counter()
-    X, Y, t_eval = generate_dataset(sizes, args.train_instances, args.train_rows, args.p, rng)
+    X, Y, t_eval = generate_dataset(sizes, args.train_instances, args.train_rows, args.p, rng, progress=progress)
     t_gen = time.perf_counter() - t0
     model = Surrogate(X.shape[1])
     t0 = time.perf_counter()
     losses = model.train(X, Y, epochs=args.epochs)
     t_train = time.perf_counter() - t0
+    progress(f"trained: {args.epochs} epochs in {t_train:.1f} s, loss {losses[0]:.4g} -> {losses[-1]:.4g}")
     results = []
-    for _ in range(args.test_instances):
+    for t in range(args.test_instances):
         inst = random_ising(args.test_n, rng)
         factory = lambda insts, p: QaoaBatch(insts, p)  # noqa: E731
         e_plain, _, ev_plain, _ = plain_optimize(factory, inst, [1.0] * args.p, [1.0] * args.p)
         e_learn, (g, b), ev_learn = learned_search(model, inst, args.p, rng)
         results.append({"plain_E": e_plain, "plain_evals": ev_plain, "learned_E": e_learn, "learned_evals": ev_learn,
                         "within_1e-3": e_learn <= e_plain + 1e-3, "eval_ratio": ev_plain / ev_learn})
+        progress(f"test instance {t}: {json.dumps(results[-1])}")
     print(json.dumps({"rows": int(X.shape[0]), "gen_s": round(t_gen, 1), "gpu_eval_s": round(t_eval, 1),
                       "train_s": round(t_train, 1), "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
                       "results": results}))

@@ -408,3 +408,33 @@ def test_precision_config2_structure():
     assert out["max_abs_error"] < 1e-5
     assert out["infidelity"] < 1e-8
     assert abs(out["c64_norm2_fp64"] - 1) < 1e-5
+
+
+def test_clone_survives_overwrite_of_source():
+    """Clone (qcs_copy_state) on a busy 27-qubit register, source reset
+    right after: the copy holds the pre-reset state."""
+    n = 27
+    reg = QubitRegister(n)
+    for q in range(n):
+        reg.ApplyGate(G.HadamardGate(), q)
+    c = reg.Clone()
+    reg.setToBasisState(3)
+    amp = 2.0 ** (-n / 2)
+    assert abs(c.getBasisStateAmplitude(0) - amp) < 1e-12
+    assert abs(c.getBasisStateAmplitude((1 << n) - 1) - amp) < 1e-12
+    assert abs(c.norm2() - 1) < 1e-10
+    assert abs(reg.getBasisStateAmplitude(3) - 1) < 1e-15
+    c.close()
+    reg.close()
+
+
+def test_fused_matches_unfused_long_circuit():
+    """The config 1 circuit at 24 qubits (a long queue of launches on the
+    register's stream): fused widths 4 and 5 agree with the unfused run."""
+    import bench
+
+    out = bench.gpu_circuit(24, 6)
+    print(out)
+    for k in ("fused4", "fused5"):
+        assert out[k]["max_abs_vs_unfused"] < 1e-12, out[k]
+        assert out[k]["infidelity_vs_unfused"] < 1e-12, out[k]
