@@ -23,6 +23,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per k_1q launch on the 30-qubit complex128 state, measured with
+# rocprofv3 --pmc FETCH_SIZE and, in a separate run, WRITE_SIZE (profiles/
+# k_1q_sweep30_c128.md): 16,777,246 KiB fetched and 16,780,000 KiB written
+# per dispatch after calibration against kernels of known traffic (k_sum_sq
+# reads 16 GiB: FETCH_SIZE x2.000; k_fill writes 16 GiB: WRITE_SIZE x1.000).
+K1Q_30Q_TRAFFIC_BYTES = (16_777_246 + 16_780_000) * 1024
 
 
 def algorithmic_bytes_1q(n, amp_bytes=16):
@@ -218,6 +224,91 @@ def gpu_precision_c2(n, layers, width=5):
     return out
 
 
+def qaoa_instances_c3(ninst, seed=31, sizes=(12, 14, 16, 18, 20, 22, 24)):
+    """Config 3 instances: n uniform over the even sizes 12..24 (a 3-regular
+    graph needs even n), J ~ N(0, 1) on edges, h ~ N(0, 0.5^2)."""
+    from qcsim_amd.qaoa import random_ising
+
+    rng = np.random.default_rng(seed)
+    return [random_ising(int(rng.choice(sizes)), rng) for _ in range(ninst)], rng
+
+
+def qaoa_eval_bytes(n, p, tile_bits, group):
+    """HBM bytes of one evaluation of an n-qubit instance (complex64, 8 B per
+    amplitude) under the stage schedule: n <= tile bits: one LDS-resident
+    pass writing the final state; else per layer a tile pass (cost + Rx on
+    the low qubits; layer 0 writes |+> without reading) and ceil((n - T)/G)
+    register-group passes, the energy read fused into the last one."""
+    A = 1 << n
+    if n <= tile_bits:
+        return 8 * A, 1
+    ng = -(-(n - tile_bits) // group)
+    passes = p * (1 + ng)
+    return 16 * A * passes - 8 * A, passes
+
+
+def gpu_qaoa_c3(ninst, p=4, steps=3):
+    """Config 3: `ninst` instances in one batch, p = 4, angles uniform in
+    [0, pi); tile bits and group width from the autotuner."""
+    from qcsim_amd.qaoa import QaoaBatch
+
+    insts, rng = qaoa_instances_c3(ninst)
+    G = rng.uniform(0, np.pi, (ninst, p))
+    B = rng.uniform(0, np.pi, (ninst, p))
+    batch = QaoaBatch(insts, p)
+    tune = batch.autotune(G, B)
+    info = batch.info()
+    ms = []
+    for _ in range(steps):
+        E = batch.evaluate(G, B)
+        ms.append(batch.last_ms())
+    batch.close()
+    t = float(np.median(ms)) / 1e3
+    tot_bytes, passes = 0, {}
+    for I in insts:
+        b, ps = qaoa_eval_bytes(I.n, p, info["tile_bits"], info["group"])
+        tot_bytes += b
+        passes[I.n] = ps
+    counts = {int(k): int(v) for k, v in zip(*np.unique([I.n for I in insts], return_counts=True))}
+    return {
+        "instances": ninst,
+        "p": p,
+        "qubits_count": counts,
+        "state_GiB": round(sum(8 << I.n for I in insts) / 2**30, 2),
+        "tile_bits": info["tile_bits"],
+        "group": info["group"],
+        "stages": info["stages"],
+        "passes_per_eval": passes,
+        "batch_ms": round(t * 1e3, 3),
+        "evals_per_s": round(ninst / t, 1),
+        "aggregate_GBps": round(tot_bytes / t / 1e9, 1),
+        "energy_mean": float(np.mean(E)),
+        "autotune_ms": tune["ms"],
+    }
+
+
+def cpu_qaoa_c3(p=4, seed=31):
+    """The oracle's QAOA circuit (gate by gate, FP64) on one instance of each
+    config 3 size; the per-size times weighted by the batch's size mix give
+    the CPU estimate of one batch evaluation."""
+    from oracle.oracle import OracleRegister, lib
+    from qcsim_amd.qaoa import random_ising
+
+    rng = np.random.default_rng(seed)
+    per_n = {}
+    for n in (12, 14, 16, 18, 20, 22, 24):
+        I = random_ising(n, rng)
+        g = rng.uniform(0, np.pi, p)
+        b = rng.uniform(0, np.pi, p)
+        r = OracleRegister(n)
+        t0 = time.perf_counter()
+        r.qaoa_exec(I.edges, I.J, I.h, g, b)
+        r.ising_energy(I.edges, I.J, I.h)
+        per_n[n] = time.perf_counter() - t0
+        r.close()
+    return per_n, int(lib().orc_max_threads())
+
+
 def cpu_baseline(n, targets, seed=7):
     from oracle.oracle import OracleRegister, lib
     from qcsim_amd import gates as G
@@ -242,6 +333,11 @@ def cpu_baseline(n, targets, seed=7):
     }
 
 
+def progress(text):
+    """One line on stderr per finished leg (stdout carries only the result)."""
+    print(f"bench: {text}", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -253,6 +349,9 @@ def main():
     ap.add_argument("--circuit-layers", type=int, default=60, help="config 1 circuit depth (0 = skip)")
     ap.add_argument("--c2", type=int, default=0, help="run the config 2 precision leg at this many qubits (32 = full)")
     ap.add_argument("--c2-layers", type=int, default=80)
+    ap.add_argument("--c3", type=int, default=1024, help="config 3 batched QAOA instances (0 = skip)")
+    ap.add_argument("--only", choices=["qft", "circuit", "c2", "c3"],
+                    help="run one leg alone and print its JSON (for profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -262,18 +361,45 @@ def main():
         )
         sys.exit(2)
     n = args.qubits
+    if args.only:
+        legs = {
+            "qft": lambda: gpu_qft(n),
+            "circuit": lambda: gpu_circuit(n, args.circuit_layers),
+            "c2": lambda: gpu_precision_c2(args.c2 or 32, args.c2_layers),
+            "c3": lambda: gpu_qaoa_c3(args.c3),
+        }
+        print(json.dumps({"leg": args.only, "result": legs[args.only]()}))
+        return
     ev_ms, wall, norm_err, tuned = gpu_sweep(n, args.steps, args.warmup, tune=not args.no_tune)
     launches = n * args.steps
     gates_per_s = launches / (ev_ms / 1e3)
     avg_launch_s = ev_ms / 1e3 / launches
     bytes_per = algorithmic_bytes_1q(n)
     achieved = bytes_per / avg_launch_s / 1e9
+    progress(f"sweep: {gates_per_s:.1f} gates/s, {achieved:.0f} GB/s")
     qft = gpu_qft(n)
+    progress(f"qft: {qft['ms']} ms")
     circuit = gpu_circuit(n, args.circuit_layers) if args.circuit_layers > 0 else None
+    progress("circuit leg done")
     c2 = gpu_precision_c2(args.c2, args.c2_layers) if args.c2 else None
+    progress("config 2 leg done")
+    c3 = gpu_qaoa_c3(args.c3) if args.c3 else None
+    progress("config 3 leg done")
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(n, [0, 7, 15, 22, 29] if n >= 30 else list(range(0, n, max(1, n // 5))))
+        progress("cpu baseline done")
+        if c3:
+            per_n, cores = cpu_qaoa_c3()
+            est = sum(per_n[k] * v for k, v in c3["qubits_count"].items())
+            c3["cpu_baseline"] = {
+                "batch_s_estimate": round(est, 2),
+                "evals_per_s": round(c3["instances"] / est, 2),
+                "cores": cores,
+                "kind": "port",
+                "sample": "one instance per size, gate-by-gate FP64 circuit plus energy: "
+                + ", ".join(f"n={k}: {v * 1e3:.1f} ms" for k, v in per_n.items()),
+            }
     out = {
         "metric": "gate/s & HBM GB/s @30q fp64",
         "value": round(gates_per_s, 3),
@@ -299,13 +425,14 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": K1Q_30Q_TRAFFIC_BYTES if n == 30 else None,
         },
         "cpu_baseline": cpu,
         "launch_variants": tuned or "default",
         "qft": qft,
         "circuit": circuit,
         "precision_c2": c2,
+        "qaoa_c3": c3,
         "host_wall_s": round(wall, 4),
         "norm_drift": norm_err,
     }

@@ -120,17 +120,35 @@ __global__ __launch_bounds__(256) void k_dense_mfma(cv<T>* __restrict__ psi, uin
     for (int t = 0; t < NT; ++t)
 #pragma unroll
         for (int s = 0; s < NS; ++s) af[t][s] = R[(16 * t + (lane & 15)) * RD + 4 * s + (lane >> 4)];
-    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        uint64_t base = tile;
+    // The next tile's amplitudes are loaded into registers while the
+    // current one is multiplied (tiles are disjoint and owned by one block).
+    auto tile_base = [&](uint64_t tile) {
+        uint64_t b = tile;
         for (uint32_t p = 0; p < tg.npos; ++p) {
             const uint64_t low = (1ull << tg.pos[p]) - 1;
-            base = ((base & ~low) << 1) | (base & low);
+            b = ((b & ~low) << 1) | (b & low);
         }
+        return b;
+    };
+    cv<T> pre[EPT];
+    uint64_t tile = blockIdx.x, base = 0;
+    if (tile < ntiles) {
+        base = tile_base(tile);
+#pragma unroll
+        for (int j = 0; j < EPT; ++j) pre[j] = ldamp<T, true>(psi + (base | off[j]));
+    }
+    for (; tile < ntiles; tile += gridDim.x) {
+        const uint64_t cur = base;
 #pragma unroll
         for (int j = 0; j < EPT; ++j) {
-            const cv<T> v = ldamp<T, true>(psi + (base | off[j]));
-            sX[row[j] * LDX + col[j]] = v.x;
-            sX[(D + row[j]) * LDX + col[j]] = v.y;
+            sX[row[j] * LDX + col[j]] = pre[j].x;
+            sX[(D + row[j]) * LDX + col[j]] = pre[j].y;
+        }
+        const uint64_t next = tile + gridDim.x;
+        if (next < ntiles) {
+            base = tile_base(next);
+#pragma unroll
+            for (int j = 0; j < EPT; ++j) pre[j] = ldamp<T, true>(psi + (base | off[j]));
         }
         __syncthreads();
         typename Mfma<T>::acc_t acc[NT];
@@ -153,7 +171,7 @@ __global__ __launch_bounds__(256) void k_dense_mfma(cv<T>* __restrict__ psi, uin
             cv<T> v;
             v.x = sX[row[j] * LDX + col[j]];
             v.y = sX[(D + row[j]) * LDX + col[j]];
-            stamp<T, true>(psi + (base | off[j]), v);
+            stamp<T, true>(psi + (cur | off[j]), v);
         }
         __syncthreads();
     }

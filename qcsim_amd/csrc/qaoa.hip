@@ -13,16 +13,24 @@
 // E(s) = -(sum h z + sum J z z), which the reference estimates with 100000
 // shots (IsingModel.h:569-578).
 //
-// Work decomposition for an n-qubit instance with group width G:
-//   n <= Tmax: one SMALL unit runs every layer and the energy in LDS;
-//   else, with tile bits T = Tmax, per layer
-//     stage A: 2^(n-T) units, each a 2^T tile: (layer 0: |+> generated in
-//              place) cost diagonal, Rx on qubits 0..T-1 in LDS;
-//     stage B_j (j = 0 .. ceil((n-T)/G) - 1): Rx on qubits T+jG .. (the
-//              last group may be narrower) with each thread's orbit in
-//              registers; the last stage of the last layer also
-//              accumulates the energy.
-// A stage's kernel pulls units from an atomic ticket until none are left.
+// Work decomposition for an n-qubit instance, tile bits T and group width G:
+//   n <= T: one SMALL unit runs every layer and the energy with the whole
+//           state in LDS (k_qaoa_tile<true>);
+//   else per layer
+//     tile launch (k_qaoa_tile<false>): 2^(n-T) units, each a 2^T tile in
+//           LDS: (layer 0: |+> generated in place) cost diagonal, Rx on
+//           qubits 0..T-1;
+//     mixer launches (k_qaoa_mix<g, E>) j = 0 .. ceil((n-T)/G) - 1: Rx on
+//           qubits T+jG .. (the last group may be narrower), each thread one
+//           2^g-amplitude orbit in registers; the last one of the last layer
+//           also accumulates the energy (E).
+// A launch pulls units from its own atomic ticket until none are left; the
+// launches of one evaluation are captured in a hipGraph.
+//
+// C(s) and E(s) are walked in Gray-code order over the bits a thread owns:
+// one full evaluation (all sites and edges) per thread, then per step the
+// change of one flipped spin q, 2 z_q h_q and 2 z_q sum_j J_qj z_j over its
+// neighbours (per-instance adjacency lists).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -35,35 +43,87 @@ namespace qcs {
 struct QInst {
     uint64_t off;        // first amplitude in the arena
     uint32_t n, ne;      // qubits, edges
-    uint32_t eoff, hoff; // into edge / site arrays
+    uint32_t eoff, hoff; // into the edge / site arrays
+    uint32_t aoff, boff; // into the adjacency offsets (n + 1) / entries (2 ne)
     uint32_t tbits;      // tile bits T (== n for SMALL instances)
     uint32_t pslot0;     // first energy-partial slot
     uint32_t npslots;    // number of energy partials
     uint32_t pad;
 };
 
-enum : uint32_t { U_SMALL = 0, U_A = 1, U_B = 2 };
-
 struct QUnit {
     uint32_t inst;
-    uint32_t kind;   // U_* | (energy << 8) | (group width << 16)
-    uint32_t idx;    // tile (A) or 256-orbit chunk (B)
+    uint32_t idx;    // tile (tile launch) or first 256-orbit chunk (mixer)
     uint32_t layer;
-    uint32_t qlo;    // first qubit of the B group
-    uint32_t slot;   // energy partial slot
+    uint32_t qlo;    // first qubit of the mixer group
+    uint32_t slot;   // energy partial slot (mixer with energy)
+    uint32_t pad;
 };
 
 constexpr uint32_t kMaxSites = 64, kMaxEdges = 256;
+constexpr uint32_t kMixChunks = 4;  // 256-orbit chunks per mixer unit
 
-// C(s) and E(s) for one basis state.
-__device__ __forceinline__ void ising_terms(uint64_t s, uint32_t n, uint32_t ne, const double* sh, const double* sJ,
-                                            const uint8_t* si, const uint8_t* sj, double& cost, double& energy) {
-    double hz = 0, jzz = 0;
-    for (uint32_t i = 0; i < n; ++i) hz += ((s >> i) & 1) ? -sh[i] : sh[i];
-    for (uint32_t k = 0; k < ne; ++k) jzz += (((s >> si[k]) ^ (s >> sj[k])) & 1) ? -sJ[k] : sJ[k];
-    cost = jzz + 0.5 * hz;
-    energy = -(hz + jzz);
+// Instance parameters staged in LDS by every unit.
+struct IsingLds {
+    double h[kMaxSites];
+    double J[kMaxEdges];
+    double aJ[2 * kMaxEdges];
+    uint16_t aoff[kMaxSites + 1];
+    uint8_t ei[kMaxEdges], ej[kMaxEdges];
+    uint8_t aj[2 * kMaxEdges];
+};
+
+struct IsingArgsDev {
+    const double* h;
+    const double* J;
+    const uint8_t* ei;
+    const uint8_t* ej;
+    const uint16_t* aoff;
+    const uint8_t* aj;
+    const double* aJ;
+};
+
+__device__ void load_params(IsingLds& P, const QInst& I, const IsingArgsDev& g) {
+    for (uint32_t i = threadIdx.x; i < I.n; i += 256) P.h[i] = g.h[I.hoff + i];
+    for (uint32_t i = threadIdx.x; i <= I.n; i += 256) P.aoff[i] = g.aoff[I.aoff + i];
+    for (uint32_t k = threadIdx.x; k < I.ne; k += 256) {
+        P.J[k] = g.J[I.eoff + k];
+        P.ei[k] = g.ei[I.eoff + k];
+        P.ej[k] = g.ej[I.eoff + k];
+    }
+    for (uint32_t k = threadIdx.x; k < 2 * I.ne; k += 256) {
+        P.aj[k] = g.aj[I.boff + k];
+        P.aJ[k] = g.aJ[I.boff + k];
+    }
+    __syncthreads();
 }
+
+// hz = sum_i h_i z_i and jzz = sum_edges J z_i z_j of basis state s.
+__device__ __forceinline__ void ising_full(uint64_t s, uint32_t n, uint32_t ne, const IsingLds& P, double& hz,
+                                           double& jzz) {
+    hz = 0;
+    jzz = 0;
+    for (uint32_t i = 0; i < n; ++i) hz += ((s >> i) & 1) ? -P.h[i] : P.h[i];
+    for (uint32_t k = 0; k < ne; ++k) jzz += (((s >> P.ei[k]) ^ (s >> P.ej[k])) & 1) ? -P.J[k] : P.J[k];
+}
+
+// Update hz, jzz for flipping spin q of s (s is the state before the flip).
+__device__ __forceinline__ void ising_flip(uint64_t s, uint32_t q, const IsingLds& P, double& hz, double& jzz) {
+    double f = 0;
+    for (uint32_t a = P.aoff[q]; a < P.aoff[q + 1]; ++a) f += ((s >> P.aj[a]) & 1) ? -P.aJ[a] : P.aJ[a];
+    if ((s >> q) & 1) {  // z_q = -1
+        hz += 2 * P.h[q];
+        jzz += 2 * f;
+    } else {
+        hz -= 2 * P.h[q];
+        jzz -= 2 * f;
+    }
+}
+
+// C(s) = sum J z z + sum h z / 2 (the cost diagonal's exponent), E(s) =
+// -(sum h z + sum J z z).
+__device__ __forceinline__ double cost_of(double hz, double jzz) { return jzz + 0.5 * hz; }
+__device__ __forceinline__ double energy_of(double hz, double jzz) { return -(hz + jzz); }
 
 __device__ __forceinline__ float2 cost_phase(float2 a, double gamma, double cost) {
     double th = gamma * cost;
@@ -90,14 +150,24 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
     return t;
 }
 
-// In-LDS layer pieces for a tile of 2^tb amplitudes starting at index `base`.
-__device__ void lds_cost(float2* t, uint32_t tb, uint64_t base, double gamma, const QInst& I, const double* sh,
-                         const double* sJ, const uint8_t* si, const uint8_t* sj) {
+// Cost diagonal on a tile of 2^tb amplitudes (global index base + i). Thread
+// t owns elements t + 256 m (the tile bits 8..tb-1 of m), visited in Gray
+// order so that each step flips one spin.
+__device__ void lds_cost(float2* t, uint32_t tb, uint64_t base, double gamma, const QInst& I, const IsingLds& P) {
     const uint32_t C = 1u << tb;
-    for (uint32_t i = threadIdx.x; i < C; i += 256) {
-        double c, e;
-        ising_terms(base + i, I.n, I.ne, sh, sJ, si, sj, c, e);
-        t[i] = cost_phase(t[i], gamma, c);
+    if (threadIdx.x < C) {
+        const uint32_t per = C > 256 ? C >> 8 : 1;
+        uint64_t s = base + threadIdx.x;
+        double hz, jzz;
+        ising_full(s, I.n, I.ne, P, hz, jzz);
+        t[threadIdx.x] = cost_phase(t[threadIdx.x], gamma, cost_of(hz, jzz));
+        for (uint32_t m = 1; m < per; ++m) {
+            const uint32_t q = 8 + __builtin_ctz(m);
+            ising_flip(s, q, P, hz, jzz);
+            s ^= 1ull << q;
+            const uint32_t i = uint32_t(s - base);
+            t[i] = cost_phase(t[i], gamma, cost_of(hz, jzz));
+        }
     }
     __syncthreads();
 }
@@ -114,54 +184,17 @@ __device__ void lds_rx(float2* t, uint32_t tb, float c, float s) {
     }
 }
 
-// Rx(2 beta) on qubits qlo .. qlo+g-1, each thread one 2^g-amplitude orbit
-// in registers; returns this thread's share of the energy when asked.
-template <int g>
-__device__ double stage_b(float2* __restrict__ psi, const QUnit& U, const QInst& I, const double* ga, bool want_e,
-                          const double* sh, const double* sJ, const uint8_t* si, const uint8_t* sj) {
-    constexpr int D = 1 << g;
-    const uint64_t o = uint64_t(U.idx) * 256 + threadIdx.x;
-    const uint64_t norb = (1ull << I.n) >> g;
-    double acc = 0;
-    if (o >= norb) return 0;
-    const uint64_t base = insert_run(o, U.qlo, g);
-    float2 x[D];
-#pragma unroll
-    for (int c = 0; c < D; ++c) x[c] = ldamp<float, true>(psi + (base | (uint64_t(c) << U.qlo)));
-    double sb, cb;
-    sincos(ga[2 * U.layer + 1], &sb, &cb);
-    const float fc = float(cb), fs = float(sb);
-#pragma unroll
-    for (int q = 0; q < g; ++q)
-#pragma unroll
-        for (int c = 0; c < D; ++c)
-            if (!((c >> q) & 1)) rx_pair(x[c], x[c | (1 << q)], fc, fs);
-#pragma unroll
-    for (int c = 0; c < D; ++c) {
-        const uint64_t s = base | (uint64_t(c) << U.qlo);
-        if (want_e) {
-            double cc, e;
-            ising_terms(s, I.n, I.ne, sh, sJ, si, sj, cc, e);
-            const double a = x[c].x, b = x[c].y;
-            acc += (a * a + b * b) * e;
-        }
-        stamp<float, true>(psi + s, x[c]);
-    }
-    return acc;
-}
-
-template <int G>
-__global__ __launch_bounds__(256) void k_qaoa_stage(float2* __restrict__ arena, const QInst* __restrict__ insts,
-                                                    const QUnit* __restrict__ units, uint32_t nunits,
-                                                    uint32_t* __restrict__ ticket, const double* __restrict__ ang,
-                                                    uint32_t p, const double* __restrict__ hv,
-                                                    const uint8_t* __restrict__ ei, const uint8_t* __restrict__ ej,
-                                                    const double* __restrict__ Jv, double* __restrict__ partial,
-                                                    double* __restrict__ energy) {
+// Tile units (SMALL = false): one layer on a 2^T tile. SMALL units: every
+// layer and the energy of an instance that fits one tile.
+template <bool SMALL>
+__global__ __launch_bounds__(256) void k_qaoa_tile(float2* __restrict__ arena, const QInst* __restrict__ insts,
+                                                   const QUnit* __restrict__ units, uint32_t nunits,
+                                                   uint32_t* __restrict__ ticket, const double* __restrict__ ang,
+                                                   uint32_t p, IsingArgsDev ga, double* __restrict__ energy) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     float2* tile = reinterpret_cast<float2*>(smem_raw);
-    __shared__ double sh[kMaxSites], sJ[kMaxEdges], red[4];
-    __shared__ uint8_t si[kMaxEdges], sj[kMaxEdges];
+    __shared__ IsingLds P;
+    __shared__ double red[4];
     __shared__ uint32_t su;
     for (;;) {
         if (threadIdx.x == 0) su = atomicAdd(ticket, 1u);
@@ -171,39 +204,39 @@ __global__ __launch_bounds__(256) void k_qaoa_stage(float2* __restrict__ arena, 
         if (u >= nunits) break;  // every wave of the block leaves together
         const QUnit U = units[u];
         const QInst I = insts[U.inst];
-        for (uint32_t i = threadIdx.x; i < I.n; i += 256) sh[i] = hv[I.hoff + i];
-        for (uint32_t k = threadIdx.x; k < I.ne; k += 256) {
-            sJ[k] = Jv[I.eoff + k];
-            si[k] = ei[I.eoff + k];
-            sj[k] = ej[I.eoff + k];
-        }
-        __syncthreads();
+        load_params(P, I, ga);
         float2* psi = arena + I.off;
-        const uint32_t kind = U.kind & 0xff;
-        const bool want_e = (U.kind >> 8) & 1;
-        const double* ga = ang + size_t(U.inst) * p * 2;
-        if (kind == U_SMALL) {
+        const double* an = ang + size_t(U.inst) * p * 2;
+        if (SMALL) {
             const uint32_t C = 1u << I.n;
             const float amp = float(1.0 / sqrt(double(C)));
             for (uint32_t i = threadIdx.x; i < C; i += 256) tile[i] = make_float2(amp, 0.f);
             __syncthreads();
             for (uint32_t l = 0; l < p; ++l) {
-                lds_cost(tile, I.n, 0, ga[2 * l], I, sh, sJ, si, sj);
+                lds_cost(tile, I.n, 0, an[2 * l], I, P);
                 double sb, cb;
-                sincos(ga[2 * l + 1], &sb, &cb);
+                sincos(an[2 * l + 1], &sb, &cb);
                 lds_rx(tile, I.n, float(cb), float(sb));
             }
             double acc = 0;
-            for (uint32_t i = threadIdx.x; i < C; i += 256) {
-                double c, e;
-                ising_terms(i, I.n, I.ne, sh, sJ, si, sj, c, e);
-                const double x = tile[i].x, y = tile[i].y;
-                acc += (x * x + y * y) * e;
-                psi[i] = tile[i];
+            if (threadIdx.x < C) {
+                const uint32_t per = C > 256 ? C >> 8 : 1;
+                uint64_t s = threadIdx.x;
+                double hz, jzz;
+                ising_full(s, I.n, I.ne, P, hz, jzz);
+                for (uint32_t m = 0;; ++m) {
+                    const float2 x = tile[s];
+                    acc += (double(x.x) * x.x + double(x.y) * x.y) * energy_of(hz, jzz);
+                    psi[s] = x;
+                    if (m + 1 == per) break;
+                    const uint32_t q = 8 + __builtin_ctz(m + 1);
+                    ising_flip(s, q, P, hz, jzz);
+                    s ^= 1ull << q;
+                }
             }
             const double tot = block_sum(acc, red);
             if (threadIdx.x == 0) energy[U.inst] = tot;
-        } else if (kind == U_A) {
+        } else {
             const uint32_t tb = I.tbits, C = 1u << tb;
             const uint64_t base = uint64_t(U.idx) << tb;
             if (U.layer == 0) {
@@ -213,26 +246,76 @@ __global__ __launch_bounds__(256) void k_qaoa_stage(float2* __restrict__ arena, 
                 for (uint32_t i = threadIdx.x; i < C; i += 256) tile[i] = ldamp<float, true>(psi + base + i);
             }
             __syncthreads();
-            lds_cost(tile, tb, base, ga[2 * U.layer], I, sh, sJ, si, sj);
+            lds_cost(tile, tb, base, an[2 * U.layer], I, P);
             double sb, cb;
-            sincos(ga[2 * U.layer + 1], &sb, &cb);
+            sincos(an[2 * U.layer + 1], &sb, &cb);
             lds_rx(tile, tb, float(cb), float(sb));
             for (uint32_t i = threadIdx.x; i < C; i += 256) stamp<float, true>(psi + base + i, tile[i]);
-        } else {
-            const uint32_t g = (U.kind >> 16) & 0xff;
-            double acc = 0;
-            switch (g) {
-                case 1: acc = stage_b<1>(psi, U, I, ga, want_e, sh, sJ, si, sj); break;
-                case 2: acc = stage_b<2>(psi, U, I, ga, want_e, sh, sJ, si, sj); break;
-                case 3: acc = stage_b<3>(psi, U, I, ga, want_e, sh, sJ, si, sj); break;
-                case 4: acc = stage_b<G >= 4 ? 4 : G>(psi, U, I, ga, want_e, sh, sJ, si, sj); break;
-                case 5: acc = stage_b<G >= 5 ? 5 : G>(psi, U, I, ga, want_e, sh, sJ, si, sj); break;
-                default: acc = stage_b<G>(psi, U, I, ga, want_e, sh, sJ, si, sj); break;
+        }
+        __syncthreads();
+    }
+}
+
+// Mixer units: Rx(2 beta) on qubits qlo .. qlo+g-1 for kMixChunks chunks of
+// 256 orbits; with E, also this unit's share of the energy.
+template <int g, bool E>
+__global__ __launch_bounds__(256) void k_qaoa_mix(float2* __restrict__ arena, const QInst* __restrict__ insts,
+                                                  const QUnit* __restrict__ units, uint32_t nunits,
+                                                  uint32_t* __restrict__ ticket, const double* __restrict__ ang,
+                                                  uint32_t p, IsingArgsDev ga, double* __restrict__ partial) {
+    constexpr int D = 1 << g;
+    __shared__ IsingLds P;
+    __shared__ double red[4];
+    __shared__ uint32_t su;
+    for (;;) {
+        if (threadIdx.x == 0) su = atomicAdd(ticket, 1u);
+        __syncthreads();
+        const uint32_t u = su;
+        __syncthreads();
+        if (u >= nunits) break;
+        const QUnit U = units[u];
+        const QInst I = insts[U.inst];
+        if (E) load_params(P, I, ga);
+        float2* psi = arena + I.off;
+        double sb, cb;
+        sincos(ang[(size_t(U.inst) * p + U.layer) * 2 + 1], &sb, &cb);
+        const float fc = float(cb), fs = float(sb);
+        const uint64_t norb = (1ull << I.n) >> g;
+        double acc = 0;
+#pragma unroll 1
+        for (uint32_t r = 0; r < kMixChunks; ++r) {
+            const uint64_t o = (uint64_t(U.idx) + r) * 256 + threadIdx.x;
+            if (o >= norb) break;
+            const uint64_t base = insert_run(o, U.qlo, g);
+            float2 x[D];
+#pragma unroll
+            for (int c = 0; c < D; ++c) x[c] = ldamp<float, true>(psi + (base | (uint64_t(c) << U.qlo)));
+#pragma unroll
+            for (int q = 0; q < g; ++q)
+#pragma unroll
+                for (int c = 0; c < D; ++c)
+                    if (!((c >> q) & 1)) rx_pair(x[c], x[c | (1 << q)], fc, fs);
+            if (E) {
+                uint64_t s = base;
+                double hz, jzz;
+                ising_full(s, I.n, I.ne, P, hz, jzz);
+#pragma unroll
+                for (int m = 0; m < D; ++m) {
+                    if (m) {
+                        const uint32_t q = U.qlo + __builtin_ctz(uint32_t(m));
+                        ising_flip(s, q, P, hz, jzz);
+                        s ^= 1ull << q;
+                    }
+                    const int c = m ^ (m >> 1);
+                    acc += (double(x[c].x) * x[c].x + double(x[c].y) * x[c].y) * energy_of(hz, jzz);
+                }
             }
-            if (want_e) {
-                const double tot = block_sum(acc, red);
-                if (threadIdx.x == 0) partial[U.slot] = tot;
-            }
+#pragma unroll
+            for (int c = 0; c < D; ++c) stamp<float, true>(psi + (base | (uint64_t(c) << U.qlo)), x[c]);
+        }
+        if (E) {
+            const double tot = block_sum(acc, red);
+            if (threadIdx.x == 0) partial[U.slot] = tot;
         }
         __syncthreads();
     }
@@ -264,18 +347,24 @@ struct qcs_qaoa {
     int device = 0;
     uint32_t ninst = 0, p = 0;
     std::vector<uint32_t> nq, ne;
-    std::vector<uint32_t> ei_h, ej_h;
-    std::vector<double> J_h, h_h;
     uint64_t total_amps = 0;
     float2* arena = nullptr;
     QInst* d_inst = nullptr;
-    uint8_t *d_ei = nullptr, *d_ej = nullptr;
-    double *d_J = nullptr, *d_h = nullptr, *d_ang = nullptr, *d_energy = nullptr, *d_partial = nullptr;
+    uint8_t *d_ei = nullptr, *d_ej = nullptr, *d_aj = nullptr;
+    uint16_t* d_aoff = nullptr;
+    double *d_J = nullptr, *d_h = nullptr, *d_aJ = nullptr;
+    double *d_ang = nullptr, *d_energy = nullptr, *d_partial = nullptr;
     uint32_t* d_large = nullptr;
     uint32_t nlarge = 0;
-    // stage plan
-    std::vector<QUnit*> d_units;
-    std::vector<uint32_t> stage_units;
+    // launch plan: one persistent launch per entry, in order
+    enum : uint8_t { L_SMALL = 0, L_TILE = 1, L_MIX = 2 };
+    struct Launch {
+        uint8_t kind, g, energy;
+        uint32_t nunits;
+        QUnit* units;
+    };
+    std::vector<Launch> plan;
+    uint32_t nstages = 0;  // dependent steps of one evaluation
     uint32_t* d_tickets = nullptr;
     uint32_t tmax = 12, group = 6;
     int use_graph = 1;
@@ -285,12 +374,13 @@ struct qcs_qaoa {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float last_ms = 0;
     std::vector<QInst> inst_h;
+
+    IsingArgsDev args() const { return IsingArgsDev{d_h, d_J, d_ei, d_ej, d_aoff, d_aj, d_aJ}; }
 };
 
 static void free_plan(qcs_qaoa* q) {
-    for (auto* u : q->d_units) (void)hipFree(u);
-    q->d_units.clear();
-    q->stage_units.clear();
+    for (auto& L : q->plan) (void)hipFree(L.units);
+    q->plan.clear();
     if (q->d_tickets) (void)hipFree(q->d_tickets);
     q->d_tickets = nullptr;
     if (q->d_partial) (void)hipFree(q->d_partial);
@@ -304,22 +394,47 @@ static void free_plan(qcs_qaoa* q) {
 }
 
 static int launch_sequence(qcs_qaoa* q) {
-    const uint32_t nst = uint32_t(q->stage_units.size());
-    hipLaunchKernelGGL(k_qaoa_reset, dim3(1), dim3(256), 0, q->stream, q->d_tickets, nst);
-    const size_t shm = sizeof(float2) << q->tmax;
-    for (uint32_t s = 0; s < nst; ++s) {
-        const uint32_t nu = q->stage_units[s];
-        const unsigned grid = std::min<uint32_t>(nu, 2048);
-#define QCS_QS(GG)                                                                                            \
-    hipLaunchKernelGGL((k_qaoa_stage<GG>), dim3(grid), dim3(256), shm, q->stream, q->arena, q->d_inst,        \
-                       q->d_units[s], nu, q->d_tickets + s, q->d_ang, q->p, q->d_h, q->d_ei, q->d_ej, q->d_J, \
-                       q->d_partial, q->d_energy)
-        switch (q->group) {
-            case 4: QCS_QS(4); break;
-            case 5: QCS_QS(5); break;
-            default: QCS_QS(6); break;
+    const uint32_t nl = uint32_t(q->plan.size());
+    hipLaunchKernelGGL(k_qaoa_reset, dim3(1), dim3(256), 0, q->stream, q->d_tickets, nl);
+    const IsingArgsDev ga = q->args();
+    for (uint32_t k = 0; k < nl; ++k) {
+        const auto& L = q->plan[k];
+        const unsigned grid = std::min<uint32_t>(L.nunits, 2048);
+        uint32_t* tk = q->d_tickets + k;
+        if (L.kind == qcs_qaoa::L_SMALL || L.kind == qcs_qaoa::L_TILE) {
+            uint32_t tb = q->tmax;
+            if (L.kind == qcs_qaoa::L_SMALL) {
+                tb = 0;
+                for (uint32_t i = 0; i < q->ninst; ++i)
+                    if (q->nq[i] <= q->tmax) tb = std::max(tb, q->nq[i]);
+            }
+            const size_t shm = sizeof(float2) << tb;
+            if (L.kind == qcs_qaoa::L_SMALL)
+                hipLaunchKernelGGL((k_qaoa_tile<true>), dim3(grid), dim3(256), shm, q->stream, q->arena, q->d_inst,
+                                   L.units, L.nunits, tk, q->d_ang, q->p, ga, q->d_energy);
+            else
+                hipLaunchKernelGGL((k_qaoa_tile<false>), dim3(grid), dim3(256), shm, q->stream, q->arena, q->d_inst,
+                                   L.units, L.nunits, tk, q->d_ang, q->p, ga, q->d_energy);
+            continue;
         }
-#undef QCS_QS
+#define QCS_MIX(GG)                                                                                            \
+    do {                                                                                                       \
+        if (L.energy)                                                                                          \
+            hipLaunchKernelGGL((k_qaoa_mix<GG, true>), dim3(grid), dim3(256), 0, q->stream, q->arena, q->d_inst, \
+                               L.units, L.nunits, tk, q->d_ang, q->p, ga, q->d_partial);                      \
+        else                                                                                                   \
+            hipLaunchKernelGGL((k_qaoa_mix<GG, false>), dim3(grid), dim3(256), 0, q->stream, q->arena,        \
+                               q->d_inst, L.units, L.nunits, tk, q->d_ang, q->p, ga, q->d_partial);           \
+    } while (0)
+        switch (L.g) {
+            case 1: QCS_MIX(1); break;
+            case 2: QCS_MIX(2); break;
+            case 3: QCS_MIX(3); break;
+            case 4: QCS_MIX(4); break;
+            case 5: QCS_MIX(5); break;
+            default: QCS_MIX(6); break;
+        }
+#undef QCS_MIX
     }
     if (q->nlarge)
         hipLaunchKernelGGL(k_qaoa_energy, dim3(std::min<uint32_t>(q->nlarge, 1024)), dim3(256), 0, q->stream,
@@ -328,70 +443,82 @@ static int launch_sequence(qcs_qaoa* q) {
     return e == hipSuccess ? QCS_OK : set_error(QCS_EDEVICE, "qaoa launch failed: %s", hipGetErrorString(e));
 }
 
-// Build the stage plan for (tmax, group) and capture it.
+// Build the launch plan for (tmax, group) and capture it.
 static int build_plan(qcs_qaoa* q) {
     free_plan(q);
     const uint32_t G = q->group, Tm = q->tmax;
-    std::vector<std::vector<QUnit>> stages;
-    auto stage = [&](size_t s) -> std::vector<QUnit>& {
-        if (stages.size() <= s) stages.resize(s + 1);
-        return stages[s];
+    struct Pending {
+        uint8_t kind, g, energy;
+        std::vector<QUnit> units;
     };
+    std::vector<Pending> pend;
     std::vector<uint32_t> large;
     uint32_t pslot = 0;
-    // stage index of (layer, part): part 0 = A, part 1.. = B groups
-    uint32_t maxgroups = 0;
-    for (uint32_t i = 0; i < q->ninst; ++i)
-        if (q->nq[i] > Tm) maxgroups = std::max(maxgroups, (q->nq[i] - Tm + G - 1) / G);
-    const uint32_t per_layer = 1 + maxgroups;
+    // SMALL instances: one launch, first (independent of the rest).
     for (uint32_t i = 0; i < q->ninst; ++i) {
         QInst& I = q->inst_h[i];
-        const uint32_t n = q->nq[i];
-        if (n <= Tm) {
-            I.tbits = n;
+        if (q->nq[i] <= Tm) {
+            I.tbits = q->nq[i];
             I.pslot0 = 0;
             I.npslots = 0;
-            stage(0).push_back(QUnit{i, U_SMALL, 0, 0, 0, 0});
-            continue;
+            if (pend.empty()) pend.push_back(Pending{qcs_qaoa::L_SMALL, 0, 0, {}});
+            pend.back().units.push_back(QUnit{i, 0, 0, 0, 0, 0});
         }
-        const uint32_t ng = (n - Tm + G - 1) / G;
-        const uint32_t tb = Tm;
-        I.tbits = tb;
-        large.push_back(i);
-        const uint32_t ntiles = 1u << (n - tb);
-        const uint32_t glast = (n - Tm) - G * (ng - 1);  // width of the last group
-        const uint32_t nchunks_last = uint32_t((((1ull << n) >> glast) + 255) / 256);
+    }
+    uint32_t maxgroups = 0;
+    for (uint32_t i = 0; i < q->ninst; ++i)
+        if (q->nq[i] > Tm) {
+            maxgroups = std::max(maxgroups, (q->nq[i] - Tm + G - 1) / G);
+            large.push_back(i);
+        }
+    // energy slots: one per unit of each instance's last mixer launch
+    for (uint32_t i : large) {
+        QInst& I = q->inst_h[i];
+        const uint32_t n = q->nq[i], ng = (n - Tm + G - 1) / G;
+        const uint32_t glast = (n - Tm) - G * (ng - 1);
+        const uint64_t chunks = (((1ull << n) >> glast) + 255) / 256;
+        I.tbits = Tm;
         I.pslot0 = pslot;
-        I.npslots = nchunks_last;
-        pslot += nchunks_last;
-        for (uint32_t l = 0; l < q->p; ++l) {
-            auto& A = stage(size_t(l) * per_layer);
-            for (uint32_t t = 0; t < ntiles; ++t) A.push_back(QUnit{i, U_A, t, l, 0, 0});
-            for (uint32_t gi = 0; gi < ng; ++gi) {
-                const uint32_t gw = gi == ng - 1 ? glast : G;
-                const uint32_t nchunks = uint32_t((((1ull << n) >> gw) + 255) / 256);
-                const bool last = (l == q->p - 1) && (gi == ng - 1);
-                auto& B = stage(size_t(l) * per_layer + 1 + gi);
-                for (uint32_t c = 0; c < nchunks; ++c)
-                    B.push_back(QUnit{i, U_B | (last ? 0x100u : 0u) | (gw << 16), c, l, tb + gi * G,
-                                      last ? I.pslot0 + c : 0});
+        I.npslots = uint32_t((chunks + kMixChunks - 1) / kMixChunks);
+        pslot += I.npslots;
+    }
+    for (uint32_t l = 0; l < q->p && !large.empty(); ++l) {
+        pend.push_back(Pending{qcs_qaoa::L_TILE, 0, 0, {}});
+        for (uint32_t i : large)
+            for (uint32_t t = 0; t < (1u << (q->nq[i] - Tm)); ++t)
+                pend.back().units.push_back(QUnit{i, t, l, 0, 0, 0});
+        for (uint32_t gi = 0; gi < maxgroups; ++gi) {
+            // one launch per (width, energy) present in this step
+            const size_t first = pend.size();
+            for (uint32_t i : large) {
+                const uint32_t n = q->nq[i], ng = (n - Tm + G - 1) / G;
+                if (gi >= ng) continue;
+                const uint32_t gw = gi == ng - 1 ? (n - Tm) - G * (ng - 1) : G;
+                const uint8_t en = (l == q->p - 1) && (gi == ng - 1);
+                size_t k = first;
+                while (k < pend.size() && !(pend[k].g == gw && pend[k].energy == en)) ++k;
+                if (k == pend.size()) pend.push_back(Pending{qcs_qaoa::L_MIX, uint8_t(gw), en, {}});
+                auto& P = pend[k];
+                const uint64_t chunks = (((1ull << n) >> gw) + 255) / 256;
+                uint32_t slot = q->inst_h[i].pslot0;
+                for (uint64_t c = 0; c < chunks; c += kMixChunks)
+                    P.units.push_back(QUnit{i, uint32_t(c), l, Tm + gi * G, en ? slot++ : 0u, 0});
             }
         }
     }
+    q->nstages = uint32_t(pend.size());
     if (hipMemcpy(q->d_inst, q->inst_h.data(), sizeof(QInst) * q->ninst, hipMemcpyHostToDevice) != hipSuccess)
         return set_error(QCS_EDEVICE, "instance upload failed");
-    for (auto& st : stages) {
-        if (st.empty()) continue;
+    for (auto& P : pend) {
+        if (P.units.empty()) continue;
         QUnit* d = nullptr;
-        if (hipMalloc(reinterpret_cast<void**>(&d), sizeof(QUnit) * st.size()) != hipSuccess)
+        if (hipMalloc(reinterpret_cast<void**>(&d), sizeof(QUnit) * P.units.size()) != hipSuccess)
             return set_error(QCS_ENOMEM, "unit table");
-        if (hipMemcpy(d, st.data(), sizeof(QUnit) * st.size(), hipMemcpyHostToDevice) != hipSuccess)
+        q->plan.push_back(qcs_qaoa::Launch{P.kind, P.g, P.energy, uint32_t(P.units.size()), d});
+        if (hipMemcpy(d, P.units.data(), sizeof(QUnit) * P.units.size(), hipMemcpyHostToDevice) != hipSuccess)
             return set_error(QCS_EDEVICE, "unit upload failed");
-        q->d_units.push_back(d);
-        q->stage_units.push_back(uint32_t(st.size()));
     }
-    if (hipMalloc(reinterpret_cast<void**>(&q->d_tickets), sizeof(uint32_t) * (q->stage_units.size() + 1)) !=
-            hipSuccess ||
+    if (hipMalloc(reinterpret_cast<void**>(&q->d_tickets), sizeof(uint32_t) * (q->plan.size() + 1)) != hipSuccess ||
         hipMalloc(reinterpret_cast<void**>(&q->d_partial), sizeof(double) * (pslot + 1)) != hipSuccess ||
         hipMalloc(reinterpret_cast<void**>(&q->d_large), sizeof(uint32_t) * (large.size() + 1)) != hipSuccess)
         return set_error(QCS_ENOMEM, "qaoa scratch");
@@ -426,6 +553,9 @@ int qcs_qaoa_destroy(qcs_qaoa* q) {
     (void)hipFree(q->d_ej);
     (void)hipFree(q->d_J);
     (void)hipFree(q->d_h);
+    (void)hipFree(q->d_aoff);
+    (void)hipFree(q->d_aj);
+    (void)hipFree(q->d_aJ);
     (void)hipFree(q->d_ang);
     (void)hipFree(q->d_energy);
     if (q->ev0) (void)hipEventDestroy(q->ev0);
@@ -469,14 +599,43 @@ int qcs_qaoa_create(uint32_t ninst, const uint32_t* nqubits, const uint32_t* ned
         eo += m;
         ho += n;
     }
-    q->ei_h.assign(ei, ei + eo);
-    q->ej_h.assign(ej, ej + eo);
-    q->J_h.assign(J, J + eo);
-    q->h_h.assign(h, h + ho);
     std::vector<uint8_t> e8i(eo + 1), e8j(eo + 1);
     for (uint32_t k = 0; k < eo; ++k) {
         e8i[k] = uint8_t(ei[k]);
         e8j[k] = uint8_t(ej[k]);
+    }
+    // adjacency lists (CSR per instance): neighbours of each site with the
+    // coupling of the joining edge
+    std::vector<uint16_t> aoff(ho + ninst + 1, 0);
+    std::vector<uint8_t> aj(2 * eo + 1, 0);
+    std::vector<double> aJ(2 * eo + 1, 0.0);
+    uint32_t ao = 0;
+    for (uint32_t i = 0; i < ninst; ++i) {
+        QInst& I = q->inst_h[i];
+        const uint32_t n = I.n, m = I.ne, e0 = I.eoff;
+        I.aoff = ao;
+        I.boff = 2 * e0;
+        std::vector<uint32_t> deg(n + 1, 0);
+        for (uint32_t k = 0; k < m; ++k) {
+            ++deg[ei[e0 + k]];
+            ++deg[ej[e0 + k]];
+        }
+        uint32_t acc = 0;
+        for (uint32_t v = 0; v < n; ++v) {
+            aoff[ao + v] = uint16_t(acc);
+            acc += deg[v];
+        }
+        aoff[ao + n] = uint16_t(acc);
+        std::vector<uint32_t> fill(n, 0);
+        for (uint32_t k = 0; k < m; ++k) {
+            const uint32_t a = ei[e0 + k], b = ej[e0 + k];
+            const uint32_t pa = I.boff + aoff[ao + a] + fill[a]++, pb = I.boff + aoff[ao + b] + fill[b]++;
+            aj[pa] = uint8_t(b);
+            aJ[pa] = J[e0 + k];
+            aj[pb] = uint8_t(a);
+            aJ[pb] = J[e0 + k];
+        }
+        ao += n + 1;
     }
     auto al = [](void** p, size_t b) { return hipMalloc(p, b ? b : 8) == hipSuccess; };
     if (!al(reinterpret_cast<void**>(&q->arena), sizeof(float2) * q->total_amps) ||
@@ -484,6 +643,9 @@ int qcs_qaoa_create(uint32_t ninst, const uint32_t* nqubits, const uint32_t* ned
         !al(reinterpret_cast<void**>(&q->d_ei), eo + 1) || !al(reinterpret_cast<void**>(&q->d_ej), eo + 1) ||
         !al(reinterpret_cast<void**>(&q->d_J), sizeof(double) * (eo + 1)) ||
         !al(reinterpret_cast<void**>(&q->d_h), sizeof(double) * (ho + 1)) ||
+        !al(reinterpret_cast<void**>(&q->d_aoff), sizeof(uint16_t) * aoff.size()) ||
+        !al(reinterpret_cast<void**>(&q->d_aj), aj.size()) ||
+        !al(reinterpret_cast<void**>(&q->d_aJ), sizeof(double) * aJ.size()) ||
         !al(reinterpret_cast<void**>(&q->d_ang), sizeof(double) * 2 * p * ninst) ||
         !al(reinterpret_cast<void**>(&q->d_energy), sizeof(double) * ninst)) {
         (void)hipGetLastError();
@@ -494,6 +656,9 @@ int qcs_qaoa_create(uint32_t ninst, const uint32_t* nqubits, const uint32_t* ned
         hipMemcpy(q->d_ej, e8j.data(), eo + 1, hipMemcpyHostToDevice) != hipSuccess ||
         (eo && hipMemcpy(q->d_J, J, sizeof(double) * eo, hipMemcpyHostToDevice) != hipSuccess) ||
         hipMemcpy(q->d_h, h, sizeof(double) * ho, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(q->d_aoff, aoff.data(), sizeof(uint16_t) * aoff.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(q->d_aj, aj.data(), aj.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(q->d_aJ, aJ.data(), sizeof(double) * aJ.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipStreamCreateWithFlags(&q->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&q->ev0) != hipSuccess || hipEventCreate(&q->ev1) != hipSuccess) {
         qcs_qaoa_destroy(q);
@@ -570,7 +735,7 @@ int qcs_qaoa_get_state(qcs_qaoa* q, uint32_t inst, double* amps) {
 
 int qcs_qaoa_info(qcs_qaoa* q, uint32_t* stages, uint64_t* amplitudes, uint32_t* tile_bits, uint32_t* group) {
     if (!q) return set_error(QCS_EINVAL, "null batch");
-    if (stages) *stages = uint32_t(q->stage_units.size());
+    if (stages) *stages = uint32_t(q->plan.size());
     if (amplitudes) *amplitudes = q->total_amps;
     if (tile_bits) *tile_bits = q->tmax;
     if (group) *group = q->group;

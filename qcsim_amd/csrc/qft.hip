@@ -105,11 +105,27 @@ struct QftArgs {
     uint32_t sq, eq, lo, g;  // range and this pass's targets [lo, lo+g-1]
 };
 
+// exp(i pi j / 32), j < 32: the phases between targets of one group, passed
+// as a kernel argument so that every lookup is a uniform constant.
+struct QftRoots {
+    double2 w[1 << kGroupMax];
+};
+
+static QftRoots group_roots() {
+    QftRoots q;
+    const long double pi = 3.141592653589793238462643383279502884L;
+    for (int j = 0; j < (1 << kGroupMax); ++j) {
+        const long double a = pi * j / (1 << kGroupMax);
+        q.w[j] = make_double2(double(std::cos(a)), double(std::sin(a)));
+    }
+    return q;
+}
+
 // D_(t+1) (or its conjugate) on local element c of an orbit whose first
 // index is `base`; the group occupies bits [lo, lo+G).
 template <typename T, int G>
 __device__ __forceinline__ void apply_d(cv<T>* x, uint64_t base, uint32_t t, const QftArgs& a,
-                                        const double2* __restrict__ tw, bool conj) {
+                                        const double2* __restrict__ tw, const QftRoots& R, bool conj) {
     constexpr int D = 1 << G;
     const uint32_t k = t + 1 - a.sq;             // phase denominator exponent
     const uint32_t nlow = a.lo - a.sq;            // fixed bits sq..lo-1 below the group
@@ -122,8 +138,9 @@ __device__ __forceinline__ void apply_d(cv<T>* x, uint64_t base, uint32_t t, con
     for (int c = 0; c < D; ++c) {
         if (!top_fixed && !((c >> (tl + 1)) & 1)) continue;
         const uint64_t Gt = uint64_t(c) & ((2ull << tl) - 1);  // bits lo..t
-        // exp(i pi (Gt 2^nlow + Lf) / 2^k) = exp(i pi Gt / 2^(t+1-lo)) * F
-        const double2 p = phase_pi(tw, Gt, tl + 1);
+        // exp(i pi (Gt 2^nlow + Lf) / 2^k) = exp(i pi Gt / 2^(t+1-lo)) * F,
+        // and exp(i pi Gt / 2^(tl+1)) = R.w[Gt << (kGroupMax - 1 - tl)]
+        const double2 p = R.w[Gt << (kGroupMax - 1 - tl)];
         double2 f;
         f.x = p.x * F.x - p.y * F.y;
         f.y = p.x * F.y + p.y * F.x;
@@ -141,7 +158,7 @@ __device__ __forceinline__ void apply_h(cv<T>* x, uint32_t tl) {
 
 template <typename T, int G, bool INV>
 __global__ __launch_bounds__(256) void k_qft_group(cv<T>* __restrict__ psi, uint64_t norb, QftArgs a,
-                                                   const double2* __restrict__ tw) {
+                                                   const double2* __restrict__ tw, const QftRoots R) {
     constexpr int D = 1 << G;
     const uint64_t o = uint64_t(blockIdx.x) * 256 + threadIdx.x;
     if (o >= norb) return;
@@ -153,7 +170,7 @@ __global__ __launch_bounds__(256) void k_qft_group(cv<T>* __restrict__ psi, uint
 #pragma unroll
         for (int tl = G - 1; tl >= 0; --tl) {
             const uint32_t t = a.lo + tl;
-            if (t < a.eq) apply_d<T, G>(x, base, t, a, tw, false);
+            if (t < a.eq) apply_d<T, G>(x, base, t, a, tw, R, false);
             apply_h<T, G>(x, tl);
         }
     } else {
@@ -161,7 +178,7 @@ __global__ __launch_bounds__(256) void k_qft_group(cv<T>* __restrict__ psi, uint
         for (int tl = 0; tl < G; ++tl) {
             const uint32_t t = a.lo + tl;
             apply_h<T, G>(x, tl);
-            if (t < a.eq) apply_d<T, G>(x, base, t, a, tw, true);
+            if (t < a.eq) apply_d<T, G>(x, base, t, a, tw, R, true);
         }
     }
 #pragma unroll
@@ -310,13 +327,16 @@ static int launch_group(qcs_register* r, const double2* tw, uint32_t sq, uint32_
     const uint64_t norb = r->A >> g;
     const unsigned grid = unsigned((norb + 255) / 256);
     const QftArgs a{sq, eq, lo, g};
+    static const QftRoots roots = group_roots();
     auto* psi = static_cast<cv<T>*>(r->psi);
-#define QCS_G(GG)                                                                                              \
-    do {                                                                                                       \
-        if (inv)                                                                                               \
-            hipLaunchKernelGGL((k_qft_group<T, GG, true>), dim3(grid), dim3(256), 0, r->stream, psi, norb, a, tw); \
-        else                                                                                                   \
-            hipLaunchKernelGGL((k_qft_group<T, GG, false>), dim3(grid), dim3(256), 0, r->stream, psi, norb, a, tw); \
+#define QCS_G(GG)                                                                                             \
+    do {                                                                                                      \
+        if (inv)                                                                                              \
+            hipLaunchKernelGGL((k_qft_group<T, GG, true>), dim3(grid), dim3(256), 0, r->stream, psi, norb, a, \
+                               tw, roots);                                                                    \
+        else                                                                                                  \
+            hipLaunchKernelGGL((k_qft_group<T, GG, false>), dim3(grid), dim3(256), 0, r->stream, psi, norb, a, \
+                               tw, roots);                                                                    \
     } while (0)
     switch (g) {
         case 1: QCS_G(1); break;

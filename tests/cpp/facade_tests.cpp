@@ -13,6 +13,7 @@
 #include <string>
 #include <vector>
 
+#include "IsingModel.h"
 #include "QuantumAlgorithm.h"
 #include "QuantumFourierTransform.h"
 #include "QuantumGate.h"
@@ -283,6 +284,40 @@ static bool samplingAndOperators() {
     return approx(reg.getBasisStateAmplitude(0), cd(0, 1. / std::sqrt(2.)));
 }
 
+// IsingTests.cpp:6-42 (IsingModelTest): 3 spins with fields, p = 3 and the
+// extra mixing; the state QAOA picks must be the minimum-energy state.
+static bool isingModelQaoa() {
+    QC::QubitRegister<> reg(3);
+    Models::QAOAIsingSubalgorithm<> ising;
+    ising.Set({{0, 1, 1.5}, {1, 2, 2}, {0, 2, -3}}, {1.25, 3.25, -2.5});
+    ising.SetP(3);
+    ising.SetGammaStart(1);
+    ising.SetBetaStart(3);
+    ising.SetGammaStart(0.3, 1);
+    ising.SetBetaStart(1, 1);
+    ising.SetMixing(true);
+    const size_t state = ising.Execute(reg);
+    return state == ising.GetMinEnergyState() && ising.GetExactEvaluations() > 0;
+}
+
+// IsingTests.cpp:45-93 (MaxCutTest): 6 nodes, 8 edges with J = -1, p = 2
+// from gamma = (6, 4.2), beta = (0.7, 0.5); the state QAOA picks must be one
+// of the minimum-energy states. The optimiser runs on the batched engine.
+static bool maxCutQaoa() {
+    QC::QubitRegister<> reg(6);
+    Models::QAOAIsingSubalgorithm<> maxcut;
+    maxcut.Set({{0, 1, -1}, {0, 2, -1}, {0, 5, -1}, {1, 2, -1}, {1, 3, -1}, {2, 3, -1}, {2, 4, -1}, {3, 5, -1}},
+               {0, 0, 0, 0, 0, 0});
+    const auto minStates = maxcut.GetMinEnergyStates();
+    maxcut.SetP(2);
+    maxcut.SetGammaStart(6);
+    maxcut.SetBetaStart(0.7);
+    maxcut.SetGammaStart(4.2, 1);
+    maxcut.SetBetaStart(0.5, 1);
+    const size_t state = maxcut.Execute(reg);
+    return minStates.count(state) == 1 && maxcut.GetExactEvaluations() % 9 == 0;
+}
+
 int main() {
     const std::vector<std::pair<const char*, bool (*)()>> tests = {
         {"measurementCollapse", measurementCollapse},
@@ -295,6 +330,8 @@ int main() {
         {"invalidArguments", invalidArguments},
         {"stateManagement", stateManagement},
         {"samplingAndOperators", samplingAndOperators},
+        {"isingModelQaoa", isingModelQaoa},
+        {"maxCutQaoa", maxCutQaoa},
     };
     int failed = 0;
     for (const auto& t : tests) {
