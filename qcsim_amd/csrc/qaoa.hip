@@ -184,6 +184,85 @@ __device__ void lds_rx(float2* t, uint32_t tb, float c, float s) {
     }
 }
 
+// LDS position of tile element i for tiles of >= 256 amplitudes: bits 0..4
+// XOR bits 5..9, which keeps every access pattern of tile_layer at the
+// minimum of two LDS cycles per wave.
+__device__ __forceinline__ uint32_t swz(uint32_t i) { return i ^ ((i >> 5) & 31u); }
+__device__ __forceinline__ uint32_t tix(uint32_t i, uint32_t tb) { return tb >= 8 ? swz(i) : i; }
+
+template <int G>
+__device__ __forceinline__ void rx_reg(float2* x, float c, float s) {
+    constexpr int D = 1 << G;
+#pragma unroll
+    for (int q = 0; q < G; ++q)
+#pragma unroll
+        for (int k = 0; k < D; ++k)
+            if (!((k >> q) & 1)) rx_pair(x[k], x[k | (1 << q)], c, s);
+}
+
+enum : int { IN_PLUS = 0, IN_GLOBAL = 1, IN_LDS = 2 };
+
+// Rx(2 beta) on the G tile qubits lo..lo+G-1, each thread taking orbits of
+// 2^G elements from the LDS tile into registers.
+template <int G>
+__device__ void rx_round(float2* t, uint32_t tb, uint32_t lo, float c, float s) {
+    constexpr int D = 1 << G;
+    for (uint32_t o = threadIdx.x; o < (1u << (tb - G)); o += 256) {
+        const uint32_t lb = uint32_t(insert_run(o, lo, G));
+        float2 y[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) y[k] = t[swz(lb + (uint32_t(k) << lo))];
+        rx_reg<G>(y, c, s);
+#pragma unroll
+        for (int k = 0; k < D; ++k) t[swz(lb + (uint32_t(k) << lo))] = y[k];
+    }
+    __syncthreads();
+}
+
+// One layer (cost diagonal, then Rx(2 beta) on qubits 0..tb-1) on a tile of
+// 2^tb amplitudes, 8 <= tb = 8 + M <= 12, whose element 0 has global index
+// `base`. Thread t takes elements t + 256 m (m < 2^M: the tile bits
+// 8..tb-1) in Gray order over m, applies the cost phase and stores them in
+// the (swizzled) LDS tile; then three rounds apply Rx on qubits 8..tb-1,
+// 4..7 and 0..3 with orbits held in registers. The input is |+> (amplitude
+// amp), `in` (global) or the LDS tile; the result is left in the LDS tile.
+template <int M>
+__device__ void tile_layer(float2* t, uint64_t base, double gamma, float c, float s, const QInst& I,
+                           const IsingLds& P, int in_mode, const float2* __restrict__ in, float amp) {
+    constexpr uint32_t tb = 8 + M;
+    const uint32_t tid = threadIdx.x;
+    uint64_t sidx = base + tid;
+    double hz, jzz;
+    ising_full(sidx, I.n, I.ne, P, hz, jzz);
+    for (uint32_t m = 0; m < (1u << M); ++m) {
+        if (m) {
+            const uint32_t q = 8 + __builtin_ctz(m);
+            ising_flip(sidx, q, P, hz, jzz);
+            sidx ^= 1ull << q;
+        }
+        const uint32_t i = uint32_t(sidx - base);
+        const float2 v = in_mode == IN_GLOBAL ? ldamp<float, true>(in + i)
+                                              : (in_mode == IN_LDS ? t[swz(i)] : make_float2(amp, 0.f));
+        t[swz(i)] = cost_phase(v, gamma, cost_of(hz, jzz));
+    }
+    __syncthreads();
+    if (M > 0) rx_round<(M > 0 ? M : 1)>(t, tb, 8, c, s);
+    rx_round<4>(t, tb, 4, c, s);
+    rx_round<4>(t, tb, 0, c, s);
+}
+
+// tile_layer for a runtime tb in 8..12.
+__device__ void tile_layer_tb(uint32_t tb, float2* t, uint64_t base, double gamma, float c, float s, const QInst& I,
+                              const IsingLds& P, int in_mode, const float2* in, float amp) {
+    switch (tb) {
+        case 8: tile_layer<0>(t, base, gamma, c, s, I, P, in_mode, in, amp); break;
+        case 9: tile_layer<1>(t, base, gamma, c, s, I, P, in_mode, in, amp); break;
+        case 10: tile_layer<2>(t, base, gamma, c, s, I, P, in_mode, in, amp); break;
+        case 11: tile_layer<3>(t, base, gamma, c, s, I, P, in_mode, in, amp); break;
+        default: tile_layer<4>(t, base, gamma, c, s, I, P, in_mode, in, amp); break;
+    }
+}
+
 // Tile units (SMALL = false): one layer on a 2^T tile. SMALL units: every
 // layer and the energy of an instance that fits one tile.
 template <bool SMALL>
@@ -210,13 +289,22 @@ __global__ __launch_bounds__(256) void k_qaoa_tile(float2* __restrict__ arena, c
         if (SMALL) {
             const uint32_t C = 1u << I.n;
             const float amp = float(1.0 / sqrt(double(C)));
-            for (uint32_t i = threadIdx.x; i < C; i += 256) tile[i] = make_float2(amp, 0.f);
-            __syncthreads();
-            for (uint32_t l = 0; l < p; ++l) {
-                lds_cost(tile, I.n, 0, an[2 * l], I, P);
-                double sb, cb;
-                sincos(an[2 * l + 1], &sb, &cb);
-                lds_rx(tile, I.n, float(cb), float(sb));
+            if (I.n >= 8) {
+                for (uint32_t l = 0; l < p; ++l) {
+                    double sb, cb;
+                    sincos(an[2 * l + 1], &sb, &cb);
+                    tile_layer_tb(I.n, tile, 0, an[2 * l], float(cb), float(sb), I, P, l ? IN_LDS : IN_PLUS,
+                                  nullptr, amp);
+                }
+            } else {
+                for (uint32_t i = threadIdx.x; i < C; i += 256) tile[i] = make_float2(amp, 0.f);
+                __syncthreads();
+                for (uint32_t l = 0; l < p; ++l) {
+                    lds_cost(tile, I.n, 0, an[2 * l], I, P);
+                    double sb, cb;
+                    sincos(an[2 * l + 1], &sb, &cb);
+                    lds_rx(tile, I.n, float(cb), float(sb));
+                }
             }
             double acc = 0;
             if (threadIdx.x < C) {
@@ -225,7 +313,7 @@ __global__ __launch_bounds__(256) void k_qaoa_tile(float2* __restrict__ arena, c
                 double hz, jzz;
                 ising_full(s, I.n, I.ne, P, hz, jzz);
                 for (uint32_t m = 0;; ++m) {
-                    const float2 x = tile[s];
+                    const float2 x = tile[tix(uint32_t(s), I.n)];
                     acc += (double(x.x) * x.x + double(x.y) * x.y) * energy_of(hz, jzz);
                     psi[s] = x;
                     if (m + 1 == per) break;
@@ -239,18 +327,20 @@ __global__ __launch_bounds__(256) void k_qaoa_tile(float2* __restrict__ arena, c
         } else {
             const uint32_t tb = I.tbits, C = 1u << tb;
             const uint64_t base = uint64_t(U.idx) << tb;
-            if (U.layer == 0) {
-                const float amp = float(1.0 / sqrt(double(1ull << I.n)));
-                for (uint32_t i = threadIdx.x; i < C; i += 256) tile[i] = make_float2(amp, 0.f);
-            } else {
-                for (uint32_t i = threadIdx.x; i < C; i += 256) tile[i] = ldamp<float, true>(psi + base + i);
-            }
-            __syncthreads();
-            lds_cost(tile, tb, base, an[2 * U.layer], I, P);
+            const float amp = float(1.0 / sqrt(double(1ull << I.n)));
             double sb, cb;
             sincos(an[2 * U.layer + 1], &sb, &cb);
-            lds_rx(tile, tb, float(cb), float(sb));
-            for (uint32_t i = threadIdx.x; i < C; i += 256) stamp<float, true>(psi + base + i, tile[i]);
+            if (tb >= 8) {
+                tile_layer_tb(tb, tile, base, an[2 * U.layer], float(cb), float(sb), I, P,
+                              U.layer ? IN_GLOBAL : IN_PLUS, psi + base, amp);
+            } else {
+                for (uint32_t i = threadIdx.x; i < C; i += 256)
+                    tile[i] = U.layer ? ldamp<float, true>(psi + base + i) : make_float2(amp, 0.f);
+                __syncthreads();
+                lds_cost(tile, tb, base, an[2 * U.layer], I, P);
+                lds_rx(tile, tb, float(cb), float(sb));
+            }
+            for (uint32_t i = threadIdx.x; i < C; i += 256) stamp<float, true>(psi + base + i, tile[tix(i, tb)]);
         }
         __syncthreads();
     }

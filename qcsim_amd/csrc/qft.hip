@@ -14,7 +14,8 @@
 //
 //   k_qft_group applies that sequence for up to 5 consecutive targets at once
 //   with the 2^5 amplitudes of each orbit in registers; k_qft_lds applies it
-//   for all targets below 12 inside 4096-amplitude chunks staged in LDS; the
+//   for all targets below 11 inside 2048-amplitude chunks staged in LDS, in
+//   rounds of up to 3 targets held in registers; the
 //   SWAP network is one bit-reversal pass (k_bitrev_tile, LDS-tiled, or
 //   k_bitrev_pairs when runs of >= 8 amplitudes stay contiguous).
 //
@@ -23,6 +24,7 @@
 // long double; the kernels need no transcendental functions.
 // qft_sequence() issues the reference's exact gate sequence instead.
 #include <cmath>
+#include <cstdlib>
 #include <complex>
 #include <mutex>
 #include <vector>
@@ -34,7 +36,8 @@ namespace qcs {
 constexpr int kTwBits = 12;
 constexpr int kTwLen = 1 << kTwBits;
 constexpr int kPhaseBits = 36;  // phases are multiples of pi / 2^36
-constexpr int kLdsLog = 12;     // k_qft_lds chunk: 4096 amplitudes
+constexpr int kLdsLog = 11;     // k_qft_lds chunk: 2048 amplitudes
+constexpr int kLdsRound = 3;    // targets per k_qft_lds round (8 amplitudes per thread)
 constexpr int kGroupMax = 5;
 
 // ---- twiddle tables (process-wide, one device copy per device) -------------
@@ -163,15 +166,21 @@ __global__ __launch_bounds__(256) void k_qft_group(cv<T>* __restrict__ psi, uint
     const uint64_t o = uint64_t(blockIdx.x) * 256 + threadIdx.x;
     if (o >= norb) return;
     const uint64_t base = insert_run(o, a.lo, G);
+    // orbit element c sits at base + c 2^lo (bits lo..lo+G-1 of base are 0)
+    cv<T>* const p0 = psi + base;
+    const uint64_t stride = 1ull << a.lo;
     cv<T> x[D];
 #pragma unroll
-    for (int c = 0; c < D; ++c) x[c] = ldamp<T, true>(psi + (base | (uint64_t(c) << a.lo)));
+    for (int c = 0; c < D; ++c) x[c] = ldamp<T, true>(p0 + c * stride);
     if (!INV) {
 #pragma unroll
         for (int tl = G - 1; tl >= 0; --tl) {
             const uint32_t t = a.lo + tl;
             if (t < a.eq) apply_d<T, G>(x, base, t, a, tw, R, false);
             apply_h<T, G>(x, tl);
+            // keep each target's phase loads next to their use (register
+            // pressure of the 2^G-amplitude orbit)
+            __builtin_amdgcn_sched_barrier(0);
         }
     } else {
 #pragma unroll
@@ -179,55 +188,111 @@ __global__ __launch_bounds__(256) void k_qft_group(cv<T>* __restrict__ psi, uint
             const uint32_t t = a.lo + tl;
             apply_h<T, G>(x, tl);
             if (t < a.eq) apply_d<T, G>(x, base, t, a, tw, R, true);
+            __builtin_amdgcn_sched_barrier(0);
         }
     }
 #pragma unroll
-    for (int c = 0; c < D; ++c) stamp<T, true>(psi + (base | (uint64_t(c) << a.lo)), x[c]);
+    for (int c = 0; c < D; ++c) stamp<T, true>(p0 + c * stride, x[c]);
 }
 
-// Targets sq..hi (hi < clog) inside contiguous chunks of 2^clog amplitudes.
+// LDS position of chunk element i: bits 0..3 XOR bits 4..7, so that the
+// 16 lanes of one access phase hit different banks for every round layout.
+__device__ __forceinline__ uint32_t swz(uint32_t i) { return i ^ ((i >> 4) & 15u); }
+
+// One round of k_qft_lds: targets a.lo .. a.lo+G-1, each thread taking
+// orbits of 2^G chunk elements into registers (the k_qft_group arithmetic).
+template <typename T, int G, bool INV>
+__device__ void lds_round(cv<T>* s, uint64_t cbase, uint32_t clog, const QftArgs& a, const double2* __restrict__ tw,
+                          const QftRoots& R) {
+    constexpr int D = 1 << G;
+    const uint32_t norb = 1u << (clog - G);
+    for (uint32_t o = threadIdx.x; o < norb; o += 256) {
+        const uint32_t lb = uint32_t(insert_run(o, a.lo, G));
+        cv<T> x[D];
+#pragma unroll
+        for (int c = 0; c < D; ++c) x[c] = s[swz(lb + (uint32_t(c) << a.lo))];
+        const uint64_t base = cbase | lb;
+        if (!INV) {
+#pragma unroll
+            for (int tl = G - 1; tl >= 0; --tl) {
+                if (a.lo + tl < a.eq) apply_d<T, G>(x, base, a.lo + tl, a, tw, R, false);
+                apply_h<T, G>(x, tl);
+            }
+        } else {
+#pragma unroll
+            for (int tl = 0; tl < G; ++tl) {
+                apply_h<T, G>(x, tl);
+                if (a.lo + tl < a.eq) apply_d<T, G>(x, base, a.lo + tl, a, tw, R, true);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < D; ++c) s[swz(lb + (uint32_t(c) << a.lo))] = x[c];
+    }
+}
+
 template <typename T, bool INV>
-__global__ __launch_bounds__(256) void k_qft_lds(cv<T>* __restrict__ psi, uint32_t clog, QftArgs a, uint32_t hi,
-                                                 const double2* __restrict__ tw) {
+__device__ void lds_round_g(cv<T>* s, uint64_t cbase, uint32_t clog, const QftArgs& a, const double2* tw,
+                            const QftRoots& R) {
+    static_assert(kLdsRound <= 3, "rounds of at most 3 targets");
+    switch (a.g) {
+        case 1: lds_round<T, 1, INV>(s, cbase, clog, a, tw, R); break;
+        case 2: lds_round<T, 2, INV>(s, cbase, clog, a, tw, R); break;
+        default: lds_round<T, 3, INV>(s, cbase, clog, a, tw, R); break;
+    }
+}
+
+// Targets sq..hi (hi < clog <= kLdsLog) inside contiguous chunks of 2^clog
+// amplitudes staged in LDS, in rounds of up to kLdsRound consecutive targets
+// (from the top down for the QFT, from sq up for its inverse); one LDS round
+// trip per round instead of one per target.
+template <typename T, bool INV>
+__global__ __launch_bounds__(256) void k_qft_lds(cv<T>* __restrict__ psi, uint32_t clog, uint64_t nchunks, QftArgs a,
+                                                 uint32_t hi, const double2* __restrict__ tw, const QftRoots R) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     cv<T>* s = reinterpret_cast<cv<T>*>(smem_raw);
+    constexpr int EPT = (1 << kLdsLog) / 256;  // chunk elements per thread
     const uint32_t C = 1u << clog;
-    const uint64_t cbase = uint64_t(blockIdx.x) << clog;
-    for (uint32_t i = threadIdx.x; i < C; i += 256) s[i] = ldamp<T, true>(psi + cbase + i);
-    __syncthreads();
-    const uint32_t nst = hi - a.sq + 1;
-    for (uint32_t step = 0; step < nst; ++step) {
-        const uint32_t t = INV ? a.sq + step : hi - step;
-        const uint32_t tb = 1u << t;
-        const bool has_d = t < a.eq;
-        const uint32_t k = t + 1 - a.sq;
-        for (uint32_t p = threadIdx.x; p < C / 2; p += 256) {
-            const uint32_t i0 = ((p >> t) << (t + 1)) | (p & (tb - 1));  // bit t = 0
-            const uint32_t i1 = i0 | tb;
-            cv<T> u = s[i0], v = s[i1];
-            // bit t+1 of the full index (inside the chunk or the chunk base)
-            const uint64_t full = cbase | i0;
-            const bool dset = has_d && ((full >> (t + 1)) & 1);
-            const uint64_t L0 = (full >> a.sq) & ((1ull << (t - a.sq)) - 1);  // bits sq..t-1
-            if (!INV) {
-                if (dset) {
-                    u = mul_ph<T>(u, phase_pi(tw, L0, k), false);
-                    v = mul_ph<T>(v, phase_pi(tw, L0 | (1ull << (t - a.sq)), k), false);
-                }
-                hadamard<T>(u, v);
-            } else {
-                hadamard<T>(u, v);
-                if (dset) {
-                    u = mul_ph<T>(u, phase_pi(tw, L0, k), true);
-                    v = mul_ph<T>(v, phase_pi(tw, L0 | (1ull << (t - a.sq)), k), true);
-                }
-            }
-            s[i0] = u;
-            s[i1] = v;
+    // The blocks stride over the chunks; each loads its next chunk into
+    // registers while the current one is transformed in LDS.
+    cv<T> pre[EPT];
+    uint64_t chunk = blockIdx.x;
+    auto load = [&](uint64_t ch) {
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            const uint32_t i = threadIdx.x + 256u * k;
+            if (i < C) pre[k] = ldamp<T, true>(psi + (ch << clog) + i);
+        }
+    };
+    if (chunk < nchunks) load(chunk);
+    for (; chunk < nchunks; chunk += gridDim.x) {
+        const uint64_t cbase = chunk << clog;
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            const uint32_t i = threadIdx.x + 256u * k;
+            if (i < C) s[swz(i)] = pre[k];
         }
         __syncthreads();
+        if (chunk + gridDim.x < nchunks) load(chunk + gridDim.x);
+        if (!INV) {
+            for (int top = int(hi); top >= int(a.sq); top -= kLdsRound) {
+                QftArgs r = a;
+                r.lo = uint32_t(top - (kLdsRound - 1) > int(a.sq) ? top - (kLdsRound - 1) : int(a.sq));
+                r.g = uint32_t(top) - r.lo + 1;
+                lds_round_g<T, false>(s, cbase, clog, r, tw, R);
+                __syncthreads();
+            }
+        } else {
+            for (uint32_t lo = a.sq; lo <= hi; lo += kLdsRound) {
+                QftArgs r = a;
+                r.lo = lo;
+                r.g = hi - lo + 1 < uint32_t(kLdsRound) ? hi - lo + 1 : uint32_t(kLdsRound);
+                lds_round_g<T, true>(s, cbase, clog, r, tw, R);
+                __syncthreads();
+            }
+        }
+        for (uint32_t i = threadIdx.x; i < C; i += 256) stamp<T, true>(psi + cbase + i, s[swz(i)]);
+        __syncthreads();
     }
-    for (uint32_t i = threadIdx.x; i < C; i += 256) stamp<T, true>(psi + cbase + i, s[i]);
 }
 
 // ---- bit reversal of the range [sq, eq] ---------------------------------------
@@ -277,7 +342,9 @@ __global__ __launch_bounds__(256) void k_bitrev_tile(cv<T>* __restrict__ psi, ui
     const uint64_t mid = below_hi;
     const uint64_t mid_r = rev_bits(mid, a.midn);
     if (mid_r < mid) return;  // the partner block handles this pair
-    const uint32_t R = 1u << a.rowlog, H = 1u << a.b, E = R * H;
+    // LDS rows are padded by one element so that the transposed reads of
+    // consecutive lanes (stride of one row) fall in different banks.
+    const uint32_t R = 1u << a.rowlog, H = 1u << a.b, E = R * H, RP = R + 1, EP = RP * H;
     auto tile_base = [&](uint64_t md) {
         return (above << (a.sq + a.m)) | (md << a.rowlog);
     };
@@ -286,8 +353,8 @@ __global__ __launch_bounds__(256) void k_bitrev_tile(cv<T>* __restrict__ psi, ui
     // element e = (h, r): r = e % R (bits [0, rowlog)), h = e / R (Hi bits)
     for (uint32_t e = threadIdx.x; e < E; e += 256) {
         const uint32_t r = e & (R - 1), h = e >> a.rowlog;
-        s[e] = psi[base0 | (uint64_t(h) << hi_lo) | r];
-        if (two) s[E + e] = psi[base1 | (uint64_t(h) << hi_lo) | r];
+        s[h * RP + r] = ldamp<T, true>(psi + (base0 | (uint64_t(h) << hi_lo) | r));
+        if (two) s[EP + h * RP + r] = ldamp<T, true>(psi + (base1 | (uint64_t(h) << hi_lo) | r));
     }
     __syncthreads();
     const uint32_t lowmask = (1u << a.sq) - 1;
@@ -296,10 +363,10 @@ __global__ __launch_bounds__(256) void k_bitrev_tile(cv<T>* __restrict__ psi, ui
         const uint32_t rd = e & (R - 1), hd = e >> a.rowlog;
         const uint32_t lo_d = rd >> a.sq, out_low = rd & lowmask;
         const uint32_t lo_s = uint32_t(rev_bits(hd, a.b)), h_s = uint32_t(rev_bits(lo_d, a.b));
-        const uint32_t src = (h_s << a.rowlog) | (lo_s << a.sq) | out_low;
+        const uint32_t src = h_s * RP + ((lo_s << a.sq) | out_low);
         // tile0's destination data comes from tile1 (or itself) and vice versa
-        psi[base0 | (uint64_t(hd) << hi_lo) | rd] = two ? s[E + src] : s[src];
-        if (two) psi[base1 | (uint64_t(hd) << hi_lo) | rd] = s[src];
+        stamp<T, true>(psi + (base0 | (uint64_t(hd) << hi_lo) | rd), two ? s[EP + src] : s[src]);
+        if (two) stamp<T, true>(psi + (base1 | (uint64_t(hd) << hi_lo) | rd), s[src]);
     }
 }
 
@@ -353,13 +420,17 @@ template <typename T>
 static int launch_lds(qcs_register* r, const double2* tw, uint32_t sq, uint32_t eq, uint32_t hi, uint32_t clog,
                       bool inv) {
     const QftArgs a{sq, eq, sq, hi - sq + 1};
-    const unsigned grid = unsigned(r->A >> clog);
+    const uint64_t nchunks = r->A >> clog;
+    const unsigned grid = unsigned(nchunks < 4096 ? nchunks : 4096);
     const size_t shm = sizeof(cv<T>) << clog;
     auto* psi = static_cast<cv<T>*>(r->psi);
+    static const QftRoots roots = group_roots();
     if (inv)
-        hipLaunchKernelGGL((k_qft_lds<T, true>), dim3(grid), dim3(256), shm, r->stream, psi, clog, a, hi, tw);
+        hipLaunchKernelGGL((k_qft_lds<T, true>), dim3(grid), dim3(256), shm, r->stream, psi, clog, nchunks, a, hi, tw,
+                           roots);
     else
-        hipLaunchKernelGGL((k_qft_lds<T, false>), dim3(grid), dim3(256), shm, r->stream, psi, clog, a, hi, tw);
+        hipLaunchKernelGGL((k_qft_lds<T, false>), dim3(grid), dim3(256), shm, r->stream, psi, clog, nchunks, a, hi,
+                           tw, roots);
     return check_launch(r, "k_qft_lds");
 }
 
@@ -387,9 +458,9 @@ static int bitrev(qcs_register* r, uint32_t sq, uint32_t eq) {
     a.midlo = a.rowlog;
     a.midn = m - 2 * a.b;
     const uint64_t ntiles = r->A >> (a.rowlog + a.b);
-    const size_t E = size_t(1) << (a.rowlog + a.b);
-    hipLaunchKernelGGL((k_bitrev_tile<T>), dim3(unsigned(ntiles)), dim3(256), 2 * E * sizeof(cv<T>), r->stream, psi,
-                       ntiles, a);
+    const size_t EP = ((size_t(1) << a.rowlog) + 1) << a.b;  // padded tile
+    hipLaunchKernelGGL((k_bitrev_tile<T>), dim3(unsigned(ntiles)), dim3(256), 2 * EP * sizeof(cv<T>), r->stream,
+                       psi, ntiles, a);
     return check_launch(r, "k_bitrev_tile");
 }
 
@@ -401,11 +472,17 @@ static int qft_fused(qcs_register* r, uint32_t sq, uint32_t eq, int do_swap, int
     const uint32_t clog = r->n < uint32_t(kLdsLog) ? r->n : uint32_t(kLdsLog);
     const bool use_lds = sq < clog && (eq - sq + 1) >= 2;
     const uint32_t lds_hi = use_lds ? (eq < clog - 1 ? eq : clog - 1) : sq - 1;
-    // register groups over (lds_hi, eq], in blocks of <= 5 from the top
+    // register groups over (lds_hi, eq], in blocks of <= gmax from the top
+    // (QCS_QFT_GROUP overrides the widest group, 1..5)
+    static const int gmax = [] {
+        const char* v = std::getenv("QCS_QFT_GROUP");
+        const int g = v ? std::atoi(v) : kGroupMax;
+        return g >= 1 && g <= kGroupMax ? g : kGroupMax;
+    }();
     std::vector<std::pair<uint32_t, uint32_t>> groups;  // (lo, g)
     const uint32_t first = use_lds ? lds_hi + 1 : sq;
     for (int64_t hi = eq; hi >= int64_t(first);) {
-        const uint32_t g = uint32_t(std::min<int64_t>(kGroupMax, hi - int64_t(first) + 1));
+        const uint32_t g = uint32_t(std::min<int64_t>(gmax, hi - int64_t(first) + 1));
         groups.push_back({uint32_t(hi) - g + 1, g});
         hi -= g;
     }
