@@ -195,6 +195,87 @@ __global__ __launch_bounds__(256) void k_qft_group(cv<T>* __restrict__ psi, uint
     for (int c = 0; c < D; ++c) stamp<T, true>(p0 + c * stride, x[c]);
 }
 
+// Five targets per pass with each 32-amplitude orbit split over a lane pair:
+// lane half h holds orbit elements c + 16 h (c < 16), so a thread keeps 16
+// amplitudes in registers; the target whose bit is the pair bit (local bit
+// 4) exchanges values with the partner lane.
+template <typename T>
+__device__ __forceinline__ void apply_d_half(cv<T>* x, uint64_t base, uint32_t t, uint32_t half, const QftArgs& a,
+                                             const double2* __restrict__ tw, const QftRoots& R, bool conj) {
+    const uint32_t k = t + 1 - a.sq;
+    const uint32_t nlow = a.lo - a.sq;
+    const uint64_t Lf = nlow ? ((base >> a.sq) & ((1ull << nlow) - 1)) : 0;
+    const double2 F = phase_pi(tw, Lf, k);
+    const uint32_t tl = t - a.lo;
+    const bool top_fixed = (t + 1 >= a.lo + kGroupMax);
+    if (top_fixed && !((base >> (t + 1)) & 1)) return;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+        const uint32_t cl = uint32_t(c) | (half << 4);
+        if (!top_fixed && !((cl >> (tl + 1)) & 1)) continue;
+        const uint32_t Gt = cl & ((2u << tl) - 1);
+        const double2 p = R.w[Gt << (kGroupMax - 1 - tl)];
+        double2 f;
+        f.x = p.x * F.x - p.y * F.y;
+        f.y = p.x * F.y + p.y * F.x;
+        x[c] = mul_ph<T>(x[c], f, conj);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void h_pair(cv<T>* x, uint32_t half) {
+    const T s = T(0.70710678118654752440);
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+        const T yx = __shfl_xor(x[c].x, 1, 64), yy = __shfl_xor(x[c].y, 1, 64);
+        if (half) {
+            x[c].x = (yx - x[c].x) * s;
+            x[c].y = (yy - x[c].y) * s;
+        } else {
+            x[c].x = (x[c].x + yx) * s;
+            x[c].y = (x[c].y + yy) * s;
+        }
+    }
+}
+
+template <typename T, bool INV>
+__global__ __launch_bounds__(256) void k_qft_group5(cv<T>* __restrict__ psi, uint64_t norb, QftArgs a,
+                                                    const double2* __restrict__ tw, const QftRoots R) {
+    const uint64_t gt = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    const uint64_t o = gt >> 1;
+    const uint32_t half = threadIdx.x & 1;
+    if (o >= norb) return;  // both lanes of a pair leave together
+    const uint64_t base = insert_run(o, a.lo, kGroupMax);
+    cv<T>* const p0 = psi + base + (uint64_t(half) << (a.lo + 4));
+    const uint64_t stride = 1ull << a.lo;
+    cv<T> x[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) x[c] = ldamp<T, true>(p0 + c * stride);
+    if (!INV) {
+#pragma unroll
+        for (int tl = kGroupMax - 1; tl >= 0; --tl) {
+            const uint32_t t = a.lo + tl;
+            if (t < a.eq) apply_d_half<T>(x, base, t, half, a, tw, R, false);
+            if (tl == kGroupMax - 1)
+                h_pair<T>(x, half);
+            else
+                apply_h<T, 4>(x, tl);
+        }
+    } else {
+#pragma unroll
+        for (int tl = 0; tl < kGroupMax; ++tl) {
+            const uint32_t t = a.lo + tl;
+            if (tl == kGroupMax - 1)
+                h_pair<T>(x, half);
+            else
+                apply_h<T, 4>(x, tl);
+            if (t < a.eq) apply_d_half<T>(x, base, t, half, a, tw, R, true);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 16; ++c) stamp<T, true>(p0 + c * stride, x[c]);
+}
+
 // LDS position of chunk element i: bits 0..3 XOR bits 4..7, so that the
 // 16 lanes of one access phase hit different banks for every round layout.
 __device__ __forceinline__ uint32_t swz(uint32_t i) { return i ^ ((i >> 4) & 15u); }
@@ -332,41 +413,70 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_bitrev_tile(cv<T>* __restrict__ psi, uint64_t ntiles, BitrevArgs a) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     cv<T>* s = reinterpret_cast<cv<T>*>(smem_raw);
-    const uint64_t tile = blockIdx.x;
-    if (tile >= ntiles) return;
-    // tile id -> (outside-high bits, Mid); the tile index enumerates all bits
-    // above rowlog except the Hi bits.
-    const uint32_t hi_lo = a.sq + a.m - a.b;      // first Hi bit
-    const uint64_t below_hi = tile & ((1ull << (hi_lo - a.rowlog)) - 1);  // Mid bits
-    const uint64_t above = tile >> (hi_lo - a.rowlog);                     // bits above eq
-    const uint64_t mid = below_hi;
-    const uint64_t mid_r = rev_bits(mid, a.midn);
-    if (mid_r < mid) return;  // the partner block handles this pair
+    constexpr int EPT = 4;  // tile elements per thread (tiles of <= 1024)
+    // The tile index enumerates the Mid bits (low) and the bits above eq;
+    // a block handles the pair (Mid, rev(Mid)) once, from the smaller Mid.
+    const uint32_t hi_lo = a.sq + a.m - a.b;  // first Hi bit
+    const uint32_t midbits = hi_lo - a.rowlog;
+    auto active = [&](uint64_t tile) {
+        const uint64_t mid = tile & ((1ull << midbits) - 1);
+        return rev_bits(mid, a.midn) >= mid;
+    };
+    auto tile_base = [&](uint64_t tile, uint64_t md) {
+        return ((tile >> midbits) << (a.sq + a.m)) | (md << a.rowlog);
+    };
     // LDS rows are padded by one element so that the transposed reads of
     // consecutive lanes (stride of one row) fall in different banks.
     const uint32_t R = 1u << a.rowlog, H = 1u << a.b, E = R * H, RP = R + 1, EP = RP * H;
-    auto tile_base = [&](uint64_t md) {
-        return (above << (a.sq + a.m)) | (md << a.rowlog);
-    };
-    const uint64_t base0 = tile_base(mid), base1 = tile_base(mid_r);
-    const bool two = mid_r != mid;
-    // element e = (h, r): r = e % R (bits [0, rowlog)), h = e / R (Hi bits)
-    for (uint32_t e = threadIdx.x; e < E; e += 256) {
-        const uint32_t r = e & (R - 1), h = e >> a.rowlog;
-        s[h * RP + r] = ldamp<T, true>(psi + (base0 | (uint64_t(h) << hi_lo) | r));
-        if (two) s[EP + h * RP + r] = ldamp<T, true>(psi + (base1 | (uint64_t(h) << hi_lo) | r));
-    }
-    __syncthreads();
     const uint32_t lowmask = (1u << a.sq) - 1;
-    for (uint32_t e = threadIdx.x; e < E; e += 256) {
-        // destination (h', r') reads source (h, r) with lo' = rev(h), h' = rev(lo)
-        const uint32_t rd = e & (R - 1), hd = e >> a.rowlog;
-        const uint32_t lo_d = rd >> a.sq, out_low = rd & lowmask;
-        const uint32_t lo_s = uint32_t(rev_bits(hd, a.b)), h_s = uint32_t(rev_bits(lo_d, a.b));
-        const uint32_t src = h_s * RP + ((lo_s << a.sq) | out_low);
-        // tile0's destination data comes from tile1 (or itself) and vice versa
-        stamp<T, true>(psi + (base0 | (uint64_t(hd) << hi_lo) | rd), two ? s[EP + src] : s[src]);
-        if (two) stamp<T, true>(psi + (base1 | (uint64_t(hd) << hi_lo) | rd), s[src]);
+    // Blocks stride over the tiles; the next active pair is loaded into
+    // registers while the current one is written out.
+    cv<T> p0[EPT], p1[EPT];
+    auto load = [&](uint64_t tile) {
+        const uint64_t mid = tile & ((1ull << midbits) - 1), mid_r = rev_bits(mid, a.midn);
+        const uint64_t b0 = tile_base(tile, mid), b1 = tile_base(tile, mid_r);
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            const uint32_t e = threadIdx.x + 256u * k;
+            if (e < E) {
+                const uint64_t off = (uint64_t(e >> a.rowlog) << hi_lo) | (e & (R - 1));
+                p0[k] = ldamp<T, true>(psi + (b0 | off));
+                if (mid_r != mid) p1[k] = ldamp<T, true>(psi + (b1 | off));
+            }
+        }
+    };
+    uint64_t tile = blockIdx.x;
+    while (tile < ntiles && !active(tile)) tile += gridDim.x;
+    if (tile < ntiles) load(tile);
+    while (tile < ntiles) {
+        const uint64_t mid = tile & ((1ull << midbits) - 1), mid_r = rev_bits(mid, a.midn);
+        const bool two = mid_r != mid;
+        const uint64_t base0 = tile_base(tile, mid), base1 = tile_base(tile, mid_r);
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            const uint32_t e = threadIdx.x + 256u * k;
+            if (e < E) {
+                const uint32_t r = e & (R - 1), h = e >> a.rowlog;
+                s[h * RP + r] = p0[k];
+                if (two) s[EP + h * RP + r] = p1[k];
+            }
+        }
+        __syncthreads();
+        uint64_t next = tile + gridDim.x;
+        while (next < ntiles && !active(next)) next += gridDim.x;
+        if (next < ntiles) load(next);
+        for (uint32_t e = threadIdx.x; e < E; e += 256) {
+            // destination (h', r') reads source (h, r) with lo' = rev(h), h' = rev(lo)
+            const uint32_t rd = e & (R - 1), hd = e >> a.rowlog;
+            const uint32_t lo_d = rd >> a.sq, out_low = rd & lowmask;
+            const uint32_t lo_s = uint32_t(rev_bits(hd, a.b)), h_s = uint32_t(rev_bits(lo_d, a.b));
+            const uint32_t src = h_s * RP + ((lo_s << a.sq) | out_low);
+            // tile0's destination data comes from tile1 (or itself) and vice versa
+            stamp<T, true>(psi + (base0 | (uint64_t(hd) << hi_lo) | rd), two ? s[EP + src] : s[src]);
+            if (two) stamp<T, true>(psi + (base1 | (uint64_t(hd) << hi_lo) | rd), s[src]);
+        }
+        __syncthreads();
+        tile = next;
     }
 }
 
@@ -410,7 +520,16 @@ static int launch_group(qcs_register* r, const double2* tw, uint32_t sq, uint32_
         case 2: QCS_G(2); break;
         case 3: QCS_G(3); break;
         case 4: QCS_G(4); break;
-        default: QCS_G(5); break;
+        default: {
+            const unsigned grid5 = unsigned((2 * norb + 255) / 256);
+            if (inv)
+                hipLaunchKernelGGL((k_qft_group5<T, true>), dim3(grid5), dim3(256), 0, r->stream, psi, norb, a, tw,
+                                   roots);
+            else
+                hipLaunchKernelGGL((k_qft_group5<T, false>), dim3(grid5), dim3(256), 0, r->stream, psi, norb, a, tw,
+                                   roots);
+            break;
+        }
     }
 #undef QCS_G
     return check_launch(r, "k_qft_group");
@@ -459,8 +578,9 @@ static int bitrev(qcs_register* r, uint32_t sq, uint32_t eq) {
     a.midn = m - 2 * a.b;
     const uint64_t ntiles = r->A >> (a.rowlog + a.b);
     const size_t EP = ((size_t(1) << a.rowlog) + 1) << a.b;  // padded tile
-    hipLaunchKernelGGL((k_bitrev_tile<T>), dim3(unsigned(ntiles)), dim3(256), 2 * EP * sizeof(cv<T>), r->stream,
-                       psi, ntiles, a);
+    const unsigned grid = unsigned(ntiles < 8192 ? ntiles : 8192);
+    hipLaunchKernelGGL((k_bitrev_tile<T>), dim3(grid), dim3(256), 2 * EP * sizeof(cv<T>), r->stream, psi, ntiles,
+                       a);
     return check_launch(r, "k_bitrev_tile");
 }
 
