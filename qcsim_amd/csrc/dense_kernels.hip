@@ -12,6 +12,7 @@
 // amplitude. k_dense (matrix in LDS, orbit in registers, no MFMA) is the
 // fallback for states too small to tile.
 #include <algorithm>
+#include <cstdlib>
 #include <vector>
 
 #include "common.h"
@@ -90,7 +91,7 @@ struct TileGeom {
 
 template <typename T, int K>
 __global__ __launch_bounds__(256) void k_dense_mfma(cv<T>* __restrict__ psi, uint64_t ntiles, TileGeom tg,
-                                                    const T* __restrict__ R) {
+                                                    const T* __restrict__ R, int xcd) {
     constexpr int D = 1 << K, RD = 2 * D, NB = 64, E = D * NB, EPT = E / 256, LDX = NB + 2;
     constexpr int NT = RD / 16, NS = RD / 4;
     __shared__ __attribute__((aligned(16))) T sX[RD * LDX];
@@ -131,7 +132,9 @@ __global__ __launch_bounds__(256) void k_dense_mfma(cv<T>* __restrict__ psi, uin
         return b;
     };
     cv<T> pre[EPT];
-    uint64_t tile = blockIdx.x, base = 0;
+    // with xcd, the blocks of one XCD (dispatched round-robin over 8) take
+    // consecutive tiles
+    uint64_t tile = xcd ? block_id<true>() : blockIdx.x, base = 0;
     if (tile < ntiles) {
         base = tile_base(tile);
 #pragma unroll
@@ -212,8 +215,12 @@ static int kdense_mfma(qcs_register* r, const uint32_t* qubits, const double* ma
     }
     const uint64_t ntiles = r->A >> (K + 6);
     const unsigned grid = unsigned(ntiles < 4096 ? ntiles : 4096);
+    static const int xcd = [] {
+        const char* v = std::getenv("QCS_DENSE_XCD");  // measurement switch, default on
+        return v ? std::atoi(v) : 1;
+    }();
     hipLaunchKernelGGL((k_dense_mfma<T, K>), dim3(grid), dim3(256), 0, r->stream, static_cast<cv<T>*>(r->psi), ntiles,
-                       tg, static_cast<const T*>(dmat));
+                       tg, static_cast<const T*>(dmat), xcd);
     return check_launch(r, "k_dense_mfma");
 }
 
