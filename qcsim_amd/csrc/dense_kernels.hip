@@ -89,10 +89,12 @@ struct TileGeom {
     uint8_t role[12];  // < 0x80: local block bit; 0x80 | j: orbit bit j
 };
 
-template <typename T, int K>
+template <typename T, int K, int NC>
 __global__ __launch_bounds__(256) void k_dense_mfma(cv<T>* __restrict__ psi, uint64_t ntiles, TileGeom tg,
                                                     const T* __restrict__ R, int xcd) {
-    constexpr int D = 1 << K, RD = 2 * D, NB = 64, E = D * NB, EPT = E / 256, LDX = NB + 2;
+    // NC groups of 64 orbit columns per tile; wave w multiplies columns
+    // 64 g + 16 w .. 64 g + 16 w + 15 of every group g
+    constexpr int D = 1 << K, RD = 2 * D, NB = 64 * NC, E = D * NB, EPT = E / 256, LDX = NB + 2;
     constexpr int NT = RD / 16, NS = RD / 4;
     __shared__ __attribute__((aligned(16))) T sX[RD * LDX];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -154,20 +156,27 @@ __global__ __launch_bounds__(256) void k_dense_mfma(cv<T>* __restrict__ psi, uin
             for (int j = 0; j < EPT; ++j) pre[j] = ldamp<T, true>(psi + (base | off[j]));
         }
         __syncthreads();
-        typename Mfma<T>::acc_t acc[NT];
+        typename Mfma<T>::acc_t acc[NC][NT];
 #pragma unroll
-        for (int t = 0; t < NT; ++t) acc[t] = typename Mfma<T>::acc_t{0, 0, 0, 0};
+        for (int g = 0; g < NC; ++g)
 #pragma unroll
-        for (int s = 0; s < NS; ++s) {
-            const T b = sX[(4 * s + (lane >> 4)) * LDX + 16 * w + (lane & 15)];
+            for (int t = 0; t < NT; ++t) acc[g][t] = typename Mfma<T>::acc_t{0, 0, 0, 0};
 #pragma unroll
-            for (int t = 0; t < NT; ++t) acc[t] = Mfma<T>::step(af[t][s], b, acc[t]);
-        }
+        for (int g = 0; g < NC; ++g)
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                const T b = sX[(4 * s + (lane >> 4)) * LDX + 64 * g + 16 * w + (lane & 15)];
+#pragma unroll
+                for (int t = 0; t < NT; ++t) acc[g][t] = Mfma<T>::step(af[t][s], b, acc[g][t]);
+            }
         __syncthreads();
 #pragma unroll
-        for (int t = 0; t < NT; ++t)
+        for (int g = 0; g < NC; ++g)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) sX[(16 * t + Mfma<T>::row(lane, r)) * LDX + 16 * w + (lane & 15)] = acc[t][r];
+            for (int t = 0; t < NT; ++t)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    sX[(16 * t + Mfma<T>::row(lane, r)) * LDX + 64 * g + 16 * w + (lane & 15)] = acc[g][t][r];
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < EPT; ++j) {
@@ -182,6 +191,10 @@ __global__ __launch_bounds__(256) void k_dense_mfma(cv<T>* __restrict__ psi, uin
 
 template <typename T, int K>
 static int kdense_mfma(qcs_register* r, const uint32_t* qubits, const double* mat_host) {
+    // 64-orbit tiles (NC = 1). 128-orbit tiles (NC = 2) measured no faster
+    // for 4-qubit blocks at 30 qubits (6.77 against 6.6-6.9 ms per launch)
+    // at lower occupancy, and do not fit the register budget of 5-qubit ones.
+    constexpr int NC = 1, OB = 6;
     constexpr int D = 1 << K, RD = 2 * D;
     std::vector<T> R(size_t(RD) * RD);
     for (int i = 0; i < D; ++i)
@@ -197,36 +210,37 @@ static int kdense_mfma(qcs_register* r, const uint32_t* qubits, const double* ma
     TileGeom tg{};
     uint64_t gmask = 0;
     for (int i = 0; i < K; ++i) gmask |= 1ull << qubits[i];
-    uint32_t orb[6], no = 0;
-    for (uint32_t b = 0; no < 6; ++b)
+    uint32_t orb[OB], no = 0;
+    for (uint32_t b = 0; no < uint32_t(OB); ++b)
         if (!(gmask >> b & 1)) orb[no++] = b;
-    uint32_t all[11];
+    uint32_t all[K + OB];
     for (int i = 0; i < K; ++i) all[i] = qubits[i];
-    for (int j = 0; j < 6; ++j) all[K + j] = orb[j];
-    std::sort(all, all + K + 6);
-    tg.npos = K + 6;
+    for (int j = 0; j < OB; ++j) all[K + j] = orb[j];
+    std::sort(all, all + K + OB);
+    tg.npos = K + OB;
     for (uint32_t p = 0; p < tg.npos; ++p) {
         tg.pos[p] = uint8_t(all[p]);
         tg.role[p] = 0xff;
         for (int i = 0; i < K; ++i)
             if (qubits[i] == all[p]) tg.role[p] = uint8_t(i);
-        for (int j = 0; j < 6; ++j)
+        for (int j = 0; j < OB; ++j)
             if (orb[j] == all[p]) tg.role[p] = uint8_t(0x80 | j);
     }
-    const uint64_t ntiles = r->A >> (K + 6);
+    const uint64_t ntiles = r->A >> (K + OB);
     const unsigned grid = unsigned(ntiles < 4096 ? ntiles : 4096);
     static const int xcd = [] {
         const char* v = std::getenv("QCS_DENSE_XCD");  // measurement switch, default on
         return v ? std::atoi(v) : 1;
     }();
-    hipLaunchKernelGGL((k_dense_mfma<T, K>), dim3(grid), dim3(256), 0, r->stream, static_cast<cv<T>*>(r->psi), ntiles,
-                       tg, static_cast<const T*>(dmat), xcd);
+    hipLaunchKernelGGL((k_dense_mfma<T, K, NC>), dim3(grid), dim3(256), 0, r->stream, static_cast<cv<T>*>(r->psi),
+                       ntiles, tg, static_cast<const T*>(dmat), xcd);
     return check_launch(r, "k_dense_mfma");
 }
 
 template <typename T, int K>
 static int kdense_big(qcs_register* r, const uint32_t* qubits, const double* mat_host) {
-    if (r->n >= uint32_t(K + 6) && !r->dense_no_mfma) return kdense_mfma<T, K>(r, qubits, mat_host);
+    constexpr uint32_t kMinQubits = K + 6;  // one MFMA tile
+    if (r->n >= kMinQubits && !r->dense_no_mfma) return kdense_mfma<T, K>(r, qubits, mat_host);
     constexpr int D = 1 << K;
     // matrix converted to T, split re[] then im[], through the staging ring
     std::vector<T> hm(2 * D * D);
