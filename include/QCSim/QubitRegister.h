@@ -20,6 +20,7 @@
 #include <chrono>
 #include <climits>
 #include <cstdint>
+#include <cstring>
 #include <map>
 #include <memory>
 #include <new>
@@ -107,6 +108,8 @@ public:
     QubitRegister& operator=(const QubitRegister&) = delete;
 
     virtual ~QubitRegister() {
+        progFwd_.reset();  // programs first: they sync the register's stream
+        progAdj_.reset();
         if (saved_) qcs_destroy(saved_);
         if (h_) qcs_destroy(h_);
     }
@@ -218,9 +221,11 @@ public:
         detail::qcs_check(qcs_apply_gate(h_, uint32_t(gateQubits), detail::gate_traits(gate), uint32_t(qubit),
                                          uint32_t(controllingQubit1), uint32_t(controllingQubit2), m.data()));
         dirty_ = true;
-        if (recordGates)
+        if (recordGates) {
             computeGates.emplace_back(
                 Gates::AppliedGate<MatrixClass>(gate.getRawOperatorMatrix(), qubit, controllingQubit1, controllingQubit2));
+            ++computeGen_;
+        }
     }
 
     void ApplyGate(const Gates::AppliedGate<MatrixClass>& gate) {
@@ -254,7 +259,10 @@ public:
             VectorClass r = m * v;
             upload(r);
         }
-        if (recordGates) computeGates.emplace_back(Gates::AppliedGate<MatrixClass>(m));
+        if (recordGates) {
+            computeGates.emplace_back(Gates::AppliedGate<MatrixClass>(m));
+            ++computeGen_;
+        }
     }
 
     const VectorClass& getRegisterStorage() const {
@@ -293,11 +301,21 @@ public:
     void ComputeStart() {
         recordGates = true;
         computeGates.clear();
+        ++computeGen_;
     }
     void ComputeEnd() { recordGates = false; }
-    void ComputeClear() { computeGates.clear(); }
+    void ComputeClear() {
+        computeGates.clear();
+        ++computeGen_;
+    }
 
+    // Compute and Uncompute replay the recorded gates; a recording of at
+    // least 8 gates of <= 3 qubits runs as one compiled graph launch
+    // (qcs_program_*), compiled on first use and again after the recording
+    // changes. The gates replay as the reference's AppliedGate (generic
+    // kernels, no traits).
     void Compute() {
+        if (runProgram(false)) return;
         const bool recordSave = recordGates;
         recordGates = false;
         for (const auto& gate : computeGates) {
@@ -308,6 +326,7 @@ public:
     }
 
     void Uncompute() {
+        if (runProgram(true)) return;
         const bool recordSave = recordGates;
         recordGates = false;
         for (auto it = computeGates.crbegin(); it != computeGates.crend(); ++it) {
@@ -426,6 +445,41 @@ protected:
     bool recordGates;
 
 private:
+    struct ProgramDeleter {
+        void operator()(qcs_program* p) const { qcs_program_destroy(p); }
+    };
+    using ProgramPtr = std::unique_ptr<qcs_program, ProgramDeleter>;
+
+    bool runProgram(bool adjoint) {
+        if (computeGates.size() < 8) return false;
+        for (const auto& g : computeGates)
+            if (g.getQubitsNumber() > 3) return false;
+        ProgramPtr& prog = adjoint ? progAdj_ : progFwd_;
+        uint64_t& stamp = adjoint ? progAdjGen_ : progFwdGen_;
+        if (!prog || stamp != computeGen_ || progSize_[adjoint] != computeGates.size()) {
+            std::vector<qcs_gate> gates(computeGates.size());
+            for (size_t i = 0; i < computeGates.size(); ++i) {
+                const auto& g = computeGates[i];
+                qcs_gate& c = gates[i];
+                std::memset(&c, 0, sizeof c);
+                c.k = uint32_t(g.getQubitsNumber());
+                c.qubits[0] = uint32_t(g.getQubit1());
+                c.qubits[1] = uint32_t(g.getQubit2());
+                c.qubits[2] = uint32_t(g.getQubit3());
+                const std::vector<double> m = detail::flatten(g.getRawOperatorMatrix());
+                std::memcpy(c.mat, m.data(), m.size() * sizeof(double));
+            }
+            qcs_program* p = nullptr;
+            detail::qcs_check(qcs_program_create(h_, gates.data(), uint32_t(gates.size()), 0, adjoint ? 1 : 0, &p));
+            prog.reset(p);
+            stamp = computeGen_;
+            progSize_[adjoint] = computeGates.size();
+        }
+        detail::qcs_check(qcs_program_run(h_, prog.get()));
+        dirty_ = true;
+        return true;
+    }
+
     void seed(unsigned int addseed) {
         if (addseed == 0) {
             std::random_device rd;
@@ -493,6 +547,10 @@ private:
     qcs_register* saved_ = nullptr;
     mutable VectorClass mirror_;
     mutable bool dirty_ = true;
+    uint64_t computeGen_ = 0;  // bumped whenever the recording changes
+    ProgramPtr progFwd_, progAdj_;
+    uint64_t progFwdGen_ = 0, progAdjGen_ = 0;
+    size_t progSize_[2] = {0, 0};
 };
 
 }  // namespace QC

@@ -123,6 +123,21 @@ int qcs_apply_circuit(qcs_register* r, const qcs_gate* gates, uint32_t n,
 int qcs_fuse(const qcs_gate* gates, uint32_t n, uint32_t width,
              qcs_gate* out, uint32_t* n_out);
 
+/* Recorded-gate replay as one graph launch: replaces the gate-by-gate loops
+   of QubitRegister::Compute / Uncompute (QubitRegister.h:553-590). A
+   program is compiled for one register: the sequence is fused to blocks of
+   <= fuse_width qubits (0: no fusion, else 2..5); with adjoint != 0 it is
+   the inverse sequence (gates reversed, matrices conjugate-transposed, as
+   Uncompute applies them); its matrices are uploaded once into memory the
+   program owns and its launches are captured into a hipGraph. qcs_program_run
+   replays the graph on the register's stream. */
+typedef struct qcs_program qcs_program;
+int qcs_program_create(qcs_register* r, const qcs_gate* gates, uint32_t n,
+                       uint32_t fuse_width, int adjoint, qcs_program** out);
+int qcs_program_run(qcs_register* r, qcs_program* p);
+int qcs_program_info(const qcs_program* p, uint32_t* launches);
+int qcs_program_destroy(qcs_program* p);
+
 /* Diagonal operator on the whole register: psi[s] *= diag[s]
  * (ApplyOperatorMatrix with a diagonal matrix, QubitRegister.h:499-505). */
 int qcs_apply_diagonal(qcs_register* r, const double* diag, uint64_t n);

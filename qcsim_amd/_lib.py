@@ -55,6 +55,10 @@ SIGNATURES = {
     "qcs_apply_circuit": (_I, [_P, C.POINTER(QcsGate), _U32, _U32]),
     "qcs_apply_diagonal": (_I, [_P, _PD, _U64]),
     "qcs_fuse": (_I, [C.POINTER(QcsGate), _U32, _U32, C.POINTER(QcsGate), C.POINTER(_U32)]),
+    "qcs_program_create": (_I, [_P, C.POINTER(QcsGate), _U32, _U32, _I, C.POINTER(_P)]),
+    "qcs_program_run": (_I, [_P, _P]),
+    "qcs_program_info": (_I, [_P, _PU32]),
+    "qcs_program_destroy": (_I, [_P]),
     "qcs_qft": (_I, [_P, _U32, _U32, _I, _I]),
     "qcs_measure_all": (_I, [_P, _D, _PU64]),
     "qcs_measure": (_I, [_P, _U32, _U32, _D, _PU64]),

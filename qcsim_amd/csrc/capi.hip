@@ -60,6 +60,15 @@ __global__ void k_f2d(const float2* __restrict__ in, double2* __restrict__ out, 
 
 int stage_upload(qcs_register* r, const void* src, size_t bytes, const void** dev) {
     if (bytes > r->mat_bytes) return set_error(QCS_EINVAL, "parameter block too large");
+    if (r->sink) {  // capturing a program: the program owns the parameter memory
+        ParamSink& ps = *r->sink;
+        const size_t need = (bytes + 255) & ~size_t(255);
+        if (ps.used + need > ps.cap) return set_error(QCS_EINVAL, "program parameter space exhausted");
+        std::memcpy(ps.host + ps.used, src, bytes);
+        *dev = ps.dev + ps.used;
+        ps.used += need;
+        return QCS_OK;
+    }
     const uint32_t slot = r->mat_next++ % qcs_register::kMatSlots;
     if (hipEventSynchronize(r->mat_ev[slot]) != hipSuccess) return set_error(QCS_EDEVICE, "staging wait failed");
     char* h = static_cast<char*>(r->hmat) + slot * r->mat_bytes;

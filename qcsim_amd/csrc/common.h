@@ -98,6 +98,15 @@ __device__ __forceinline__ cv<T> cmul(T ar, T ai, cv<T> x) {
     return r;
 }
 
+// Device parameter space of a program being captured: uploads of dense
+// matrices go to dev + used (copied from the host mirror once the capture
+// ends) instead of the staging ring.
+struct ParamSink {
+    char* host = nullptr;
+    char* dev = nullptr;
+    size_t cap = 0, used = 0;
+};
+
 // Engine register: one HBM buffer of 2^n amplitudes, updated in place,
 // plus FP64 reduction scratch and its own HIP stream.
 struct qcs_register {
@@ -123,6 +132,7 @@ struct qcs_register {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     uint64_t launches = 0;
     bool dense_no_mfma = false;  // route dense blocks to the non-MFMA kernel
+    ParamSink* sink = nullptr;   // set while a program is captured
 
     size_t amp_bytes() const { return prec == QCS_COMPLEX128 ? 16 : 8; }
 };

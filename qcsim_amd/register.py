@@ -183,7 +183,8 @@ class QubitRegister:
         q = (C.c_uint32 * len(qubits))(*qubits)
         check(lib().qcs_apply_dense(self._h, len(qubits), q, _dptr(_mat_buf(m))))
 
-    def apply_circuit(self, gates, fuse_width: int = 0):
+    @staticmethod
+    def _gate_array(gates):
         arr = (QcsGate * len(gates))()
         for i, (gate, qubits) in enumerate(gates):
             m = _mat_buf(gate.getRawOperatorMatrix())
@@ -193,7 +194,16 @@ class QubitRegister:
             for j, q in enumerate(qubits):
                 g.qubits[j] = q
             C.memmove(g.mat, m.ctypes.data, m.nbytes)
+        return arr
+
+    def apply_circuit(self, gates, fuse_width: int = 0):
+        arr = self._gate_array(gates)
         check(lib().qcs_apply_circuit(self._h, arr, len(gates), fuse_width))
+
+    def compile_program(self, gates, fuse_width: int = 0, adjoint: bool = False) -> "Program":
+        """The gate list captured as a graph for this register (qcs_program_*):
+        Compute-style replay, or with adjoint=True the Uncompute order."""
+        return Program(self, gates, fuse_width, adjoint)
 
     def ApplyOperatorMatrix(self, m):
         m = np.asarray(m, dtype=np.complex128)
@@ -428,3 +438,34 @@ class QuantumFourierTransform:
                     reg.ApplyGate(self.cPhaseShift, cur, ctrl)
                     phase *= 0.5
             reg.ApplyGate(self.hadamard, eq)
+
+
+class Program:
+    """A gate sequence compiled for one register and replayed as one graph
+    launch (the recorded-gate replay of QubitRegister.h:553-590)."""
+
+    def __init__(self, reg: QubitRegister, gates, fuse_width: int = 0, adjoint: bool = False):
+        arr = QubitRegister._gate_array(gates)
+        h = C.c_void_p()
+        check(lib().qcs_program_create(reg._h, arr, len(gates), fuse_width, int(adjoint), C.byref(h)))
+        self._h = h
+        self._reg = reg
+
+    def run(self):
+        check(lib().qcs_program_run(self._reg._h, self._h))
+
+    def launches(self) -> int:
+        n = C.c_uint32()
+        check(lib().qcs_program_info(self._h, C.byref(n)))
+        return n.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().qcs_program_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -284,6 +284,45 @@ static bool samplingAndOperators() {
     return approx(reg.getBasisStateAmplitude(0), cd(0, 1. / std::sqrt(2.)));
 }
 
+// Compute / Uncompute (QubitRegister.h:536-590) repeated, with the recording
+// changed in between: the replay (one compiled graph launch for >= 8 gates)
+// follows the current recording.
+static bool computeUncomputeReplay() {
+    const size_t n = 10;
+    QC::QubitRegister<> reg(n);
+    QC::Gates::HadamardGate<> h;
+    QC::Gates::CNOTGate<> cx;
+    QC::Gates::RyGate<> ry;
+    ry.SetTheta(0.3);
+    const Eigen::VectorXcd in = reg.getRegisterStorage();
+    auto same = [&](const Eigen::VectorXcd& a, const Eigen::VectorXcd& b) {
+        for (size_t k = 0; k < (size_t(1) << n); ++k)
+            if (!approx(a(k), b(k), 1e-12)) return false;
+        return true;
+    };
+    reg.ComputeStart();
+    for (size_t q = 0; q < n; ++q) reg.ApplyGate(h, q);
+    for (size_t q = 0; q + 1 < n; ++q) reg.ApplyGate(cx, q + 1, q);
+    reg.ComputeEnd();
+    const Eigen::VectorXcd once = reg.getRegisterStorage();
+    for (int rep = 0; rep < 3; ++rep) {
+        reg.Uncompute();
+        if (!same(reg.getRegisterStorage(), in)) return false;
+        reg.Compute();
+        if (!same(reg.getRegisterStorage(), once)) return false;
+    }
+    reg.Uncompute();
+    reg.ComputeStart();
+    for (size_t q = 0; q < n; ++q) reg.ApplyGate(ry, q);
+    reg.ComputeEnd();
+    reg.Uncompute();
+    if (!same(reg.getRegisterStorage(), in)) return false;
+    reg.Compute();
+    const double c = std::cos(0.15), s = std::sin(0.15);
+    return approx(reg.getBasisStateAmplitude(0), std::pow(c, double(n)), 1e-12) &&
+           approx(reg.getBasisStateAmplitude((size_t(1) << n) - 1), std::pow(s, double(n)), 1e-12);
+}
+
 // IsingTests.cpp:6-42 (IsingModelTest): 3 spins with fields, p = 3 and the
 // extra mixing; the state QAOA picks must be the minimum-energy state.
 static bool isingModelQaoa() {
@@ -330,6 +369,7 @@ int main() {
         {"invalidArguments", invalidArguments},
         {"stateManagement", stateManagement},
         {"samplingAndOperators", samplingAndOperators},
+        {"computeUncomputeReplay", computeUncomputeReplay},
         {"isingModelQaoa", isingModelQaoa},
         {"maxCutQaoa", maxCutQaoa},
     };

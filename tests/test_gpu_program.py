@@ -1,0 +1,150 @@
+"""Compiled gate programs (qcs_program_*): the recorded-gate replay of
+QubitRegister::Compute / Uncompute (QubitRegister.h:553-590) captured as a
+hipGraph. A replay must equal the eager launches bit for bit, replays must
+stay equal with eager work between them, and the adjoint program must undo
+the forward one."""
+import numpy as np
+import pytest
+
+from gatecatalog import catalog
+from qcsim_amd import QubitRegister
+from qcsim_amd import gates as G
+from refmath import random_state
+
+pytestmark = pytest.mark.gpu
+
+
+def random_sequence(rng, n, count):
+    one, two, three = catalog(7)
+    fams = one + two + three
+    seq = []
+    for _ in range(count):
+        _, gate = fams[int(rng.integers(len(fams)))]
+        seq.append((gate, [int(x) for x in rng.choice(n, size=gate.getQubitsNumber(), replace=False)]))
+    return seq
+
+
+def fresh(n, psi, precision="c128"):
+    r = QubitRegister(n, precision=precision)
+    r.setRegisterStorageFastNoNormalize(psi.copy())
+    return r
+
+
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+@pytest.mark.parametrize("width", [0, 3, 5])
+def test_program_replays_equal_eager_launches(precision, width):
+    n = 14
+    rng = np.random.default_rng(10 + width)
+    seq = random_sequence(rng, n, 160)
+    psi = random_state(rng, n)
+    g = fresh(n, psi, precision)
+    e = fresh(n, psi, precision)
+    prog = g.compile_program(seq, fuse_width=width)
+    assert prog.launches() >= 1
+    h = G.HadamardGate()
+    for rep in range(3):
+        prog.run()
+        e.apply_circuit(seq, fuse_width=width)
+        assert np.array_equal(g.getRegisterStorage(), e.getRegisterStorage()), rep
+        # eager work between replays
+        g.ApplyGate(h, rep)
+        e.ApplyGate(h, rep)
+    prog.close()
+    g.close()
+    e.close()
+
+
+@pytest.mark.parametrize("width", [0, 4, 5])
+def test_adjoint_program_undoes_the_forward_one(width):
+    n = 13
+    rng = np.random.default_rng(20 + width)
+    seq = random_sequence(rng, n, 120)
+    psi = random_state(rng, n)
+    r = fresh(n, psi)
+    fwd = r.compile_program(seq, fuse_width=width)
+    inv = r.compile_program(seq, fuse_width=width, adjoint=True)
+    fwd.run()
+    assert np.max(np.abs(r.getRegisterStorage() - psi)) > 1e-3
+    inv.run()
+    assert np.max(np.abs(r.getRegisterStorage() - psi)) < 1e-12
+    # the adjoint program is the eager inverse sequence
+    e = fresh(n, psi)
+    fwd.run()
+    for gate, qs in reversed(seq):
+        m = gate.getRawOperatorMatrix().conj().T
+        e.ApplyGate(G.AppliedGate(m, *(qs + [0] * (3 - len(qs)))))
+    e_state = e.getRegisterStorage()
+    r2 = fresh(n, psi)
+    inv2 = r2.compile_program(seq, fuse_width=0, adjoint=True)
+    inv2.run()
+    assert np.max(np.abs(r2.getRegisterStorage() - e_state)) < 1e-12
+    for x in (fwd, inv, inv2):
+        x.close()
+    for x in (r, e, r2):
+        x.close()
+
+
+def test_program_is_bound_to_its_register():
+    from qcsim_amd._lib import check, lib
+
+    a, b = QubitRegister(6), QubitRegister(6)
+    prog = a.compile_program([(G.HadamardGate(), [0])])
+    with pytest.raises(ValueError, match="another register"):
+        check(lib().qcs_program_run(b._h, prog._h))
+    with pytest.raises(ValueError, match="out of range"):
+        a.compile_program([(G.HadamardGate(), [6])])
+    prog.close()
+    a.close()
+    b.close()
+
+
+def test_program_replay_timing_small_state():
+    """16 qubits, 600 gates: one graph launch against 600 launches (printed,
+    and the replay must not be slower)."""
+    import time
+
+    n = 16
+    rng = np.random.default_rng(5)
+    seq = [(G.SingleQubitGate(G.haar_u2(rng)), [int(rng.integers(n))]) for _ in range(600)]
+    r = QubitRegister(n)
+    prog = r.compile_program(seq)
+    for _ in range(2):
+        prog.run()
+        r.apply_circuit(seq)
+    r.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        r.apply_circuit(seq)
+    r.synchronize()
+    t_eager = (time.perf_counter() - t0) / 5
+    t0 = time.perf_counter()
+    for _ in range(5):
+        prog.run()
+    r.synchronize()
+    t_graph = (time.perf_counter() - t0) / 5
+    print(f"600 gates at 16 qubits: eager {t_eager * 1e3:.2f} ms, graph {t_graph * 1e3:.2f} ms")
+    assert t_graph < 1.2 * t_eager
+    prog.close()
+    r.close()
+
+
+def test_program_follows_state_buffer_swaps():
+    """RestoreStateDestructive exchanges the register's state buffer
+    (qcs_swap_state) and releases the other one: the program must run on the
+    buffer the register holds now."""
+    n = 12
+    rng = np.random.default_rng(31)
+    seq = random_sequence(rng, n, 60)
+    psi = random_state(rng, n)
+    r = fresh(n, psi)
+    e = fresh(n, psi)
+    prog = r.compile_program(seq, fuse_width=4)
+    r.SaveState()
+    prog.run()
+    r.RestoreStateDestructive()
+    prog.run()
+    e.apply_circuit(seq, fuse_width=4)
+    assert np.array_equal(r.getRegisterStorage(), e.getRegisterStorage())
+    prog.close()
+    r.close()
+    e.close()
