@@ -191,6 +191,7 @@ class QubitRegister:
             g = arr[i]
             g.k = gate.getQubitsNumber()
             g.traits = gate.traits
+            g.flags = getattr(gate, "flags", 0)
             for j, q in enumerate(qubits):
                 g.qubits[j] = q
             C.memmove(g.mat, m.ctypes.data, m.nbytes)

@@ -148,3 +148,27 @@ def test_program_follows_state_buffer_swaps():
     prog.close()
     r.close()
     e.close()
+
+
+def test_circuit_file_runs_on_the_engine(tmp_path):
+    """A circuit written to a binary file and read back runs bit for bit as
+    the original list, eagerly and as a program."""
+    from qcsim_amd import circuit as cf
+
+    n = 12
+    rng = np.random.default_rng(41)
+    seq = random_sequence(rng, n, 80)
+    path = str(tmp_path / "c.qcsc")
+    cf.save(path, cf.Circuit(n, seq))
+    loaded = cf.load(path)
+    psi = random_state(rng, n)
+    a, b, c = fresh(n, psi), fresh(n, psi), fresh(n, psi)
+    a.apply_circuit(seq)
+    b.apply_circuit(loaded.gates)
+    prog = c.compile_program(loaded.gates, fuse_width=4)
+    prog.run()
+    assert np.array_equal(a.getRegisterStorage(), b.getRegisterStorage())
+    assert np.max(np.abs(a.getRegisterStorage() - c.getRegisterStorage())) < 1e-12
+    prog.close()
+    for r in (a, b, c):
+        r.close()
