@@ -287,6 +287,25 @@ def gpu_qaoa_c3(ninst, p=4, steps=3):
     }
 
 
+def gpu_learned_c4():
+    """Config 4: 1e6 (instance, angles, exact energy) rows from the batched
+    engine on 16-20 qubits, the 3x256 BF16 surrogate trained by the HIP
+    kernels, top-32 proposals re-scored exactly on 8 new 20-qubit instances
+    against the plain optimiser (qcsim_amd.learned.run_experiment)."""
+    from qcsim_amd.learned import run_experiment
+
+    out = run_experiment()
+    rs = out.pop("results")
+    out["plain_evals"] = [r["plain_evals"] for r in rs]
+    out["plain_evals_to_gap"] = [r["plain_evals_to_gap"] for r in rs]
+    out["learned_evals_to_gap"] = [r["learned_evals_to_gap"] for r in rs]
+    out["eval_ratio_budget"] = [round(r["eval_ratio_budget"], 2) for r in rs]
+    out["plain_E"] = [round(r["plain_E"], 4) for r in rs]
+    out["learned_E"] = [round(r["learned_E"], 4) for r in rs]
+    out["ground_E"] = [round(r["ground_E"], 4) for r in rs]
+    return out
+
+
 def cpu_qaoa_c3(p=4, seed=31):
     """The oracle's QAOA circuit (gate by gate, FP64) on one instance of each
     config 3 size; the per-size times weighted by the batch's size mix give
@@ -350,7 +369,8 @@ def main():
     ap.add_argument("--c2", type=int, default=0, help="run the config 2 precision leg at this many qubits (32 = full)")
     ap.add_argument("--c2-layers", type=int, default=80)
     ap.add_argument("--c3", type=int, default=1024, help="config 3 batched QAOA instances (0 = skip)")
-    ap.add_argument("--only", choices=["qft", "circuit", "c2", "c3"],
+    ap.add_argument("--c4", type=int, default=1, help="config 4 learned-proposal experiment (0 = skip)")
+    ap.add_argument("--only", choices=["qft", "circuit", "c2", "c3", "c4"],
                     help="run one leg alone and print its JSON (for profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -367,6 +387,7 @@ def main():
             "circuit": lambda: gpu_circuit(n, args.circuit_layers),
             "c2": lambda: gpu_precision_c2(args.c2 or 32, args.c2_layers),
             "c3": lambda: gpu_qaoa_c3(args.c3),
+            "c4": gpu_learned_c4,
         }
         print(json.dumps({"leg": args.only, "result": legs[args.only]()}))
         return
@@ -385,6 +406,8 @@ def main():
     progress("config 2 leg done")
     c3 = gpu_qaoa_c3(args.c3) if args.c3 else None
     progress("config 3 leg done")
+    c4 = gpu_learned_c4() if args.c4 else None
+    progress("config 4 leg done")
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(n, [0, 7, 15, 22, 29] if n >= 30 else list(range(0, n, max(1, n // 5))))
@@ -433,6 +456,7 @@ def main():
         "circuit": circuit,
         "precision_c2": c2,
         "qaoa_c3": c3,
+        "learned_c4": c4,
         "host_wall_s": round(wall, 4),
         "norm_drift": norm_err,
     }

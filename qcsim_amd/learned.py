@@ -195,6 +195,37 @@ def compare_on_instance(model: Surrogate, inst: IsingInstance, p: int, rng: np.r
     return out
 
 
+def run_experiment(train_rows=1_000_000, train_instances=2048, sizes=(16, 18, 20), p=4, epochs=20, test_n=20,
+                   test_instances=8, seed=2024, top=32, progress=None):
+    """Config 4 end to end: generate the data set with the batched engine,
+    train the surrogate, compare learned proposals with the plain optimiser
+    on new instances (compare_on_instance). Returns a JSON-ready summary."""
+    rng = np.random.default_rng(seed)
+    t0 = time.perf_counter()
+    X, Y, t_eval = generate_dataset(list(sizes), train_instances, train_rows, p, rng, progress=progress)
+    t_gen = time.perf_counter() - t0
+    model = Surrogate(X.shape[1])
+    t0 = time.perf_counter()
+    losses = model.train(X, Y, epochs=epochs)
+    t_train = time.perf_counter() - t0
+    if progress:
+        progress(f"trained: {epochs} epochs in {t_train:.1f} s, loss {losses[0]:.4g} -> {losses[-1]:.4g}")
+    results = []
+    for t in range(test_instances):
+        inst = random_ising(test_n, rng)
+        results.append(compare_on_instance(model, inst, p, rng, top=top))
+        if progress:
+            progress(f"test instance {t}: {json.dumps(results[-1])}")
+    model.close()
+    ok = [r for r in results if r["within_gap"]]
+    ratios = [r["eval_ratio_to_gap"] for r in ok if r["eval_ratio_to_gap"]]
+    return {"rows": int(X.shape[0]), "gen_s": round(t_gen, 1), "gpu_eval_s": round(t_eval, 1),
+            "train_s": round(t_train, 1), "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
+            "within_gap": f"{len(ok)}/{len(results)}",
+            "median_eval_ratio_to_gap": float(np.median(ratios)) if ratios else None,
+            "results": results}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--train-rows", type=int, default=1_000_000)
@@ -207,32 +238,13 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--top", type=int, default=32)
     args = ap.parse_args()
-    rng = np.random.default_rng(args.seed)
-    sizes = [int(s) for s in args.sizes.split(",")]
 
     def progress(text):
         print(text, file=sys.stderr, flush=True)
 
-    t0 = time.perf_counter()
-    X, Y, t_eval = generate_dataset(sizes, args.train_instances, args.train_rows, args.p, rng, progress=progress)
-    t_gen = time.perf_counter() - t0
-    model = Surrogate(X.shape[1])
-    t0 = time.perf_counter()
-    losses = model.train(X, Y, epochs=args.epochs)
-    t_train = time.perf_counter() - t0
-    progress(f"trained: {args.epochs} epochs in {t_train:.1f} s, loss {losses[0]:.4g} -> {losses[-1]:.4g}")
-    results = []
-    for t in range(args.test_instances):
-        inst = random_ising(args.test_n, rng)
-        results.append(compare_on_instance(model, inst, args.p, rng, top=args.top))
-        progress(f"test instance {t}: {json.dumps(results[-1])}")
-    ok = [r for r in results if r["within_gap"]]
-    ratios = [r["eval_ratio_to_gap"] for r in ok if r["eval_ratio_to_gap"]]
-    print(json.dumps({"rows": int(X.shape[0]), "gen_s": round(t_gen, 1), "gpu_eval_s": round(t_eval, 1),
-                      "train_s": round(t_train, 1), "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
-                      "within_gap": f"{len(ok)}/{len(results)}",
-                      "median_eval_ratio_to_gap": float(np.median(ratios)) if ratios else None,
-                      "results": results}))
+    out = run_experiment(args.train_rows, args.train_instances, [int(x) for x in args.sizes.split(",")], args.p,
+                         args.epochs, args.test_n, args.test_instances, args.seed, args.top, progress)
+    print(json.dumps(out))
 
 
 if __name__ == "__main__":
