@@ -84,9 +84,12 @@ __device__ __forceinline__ void stamp(cv<T>* p, cv<T> a) {
 
 // Block index remap so that the 8 XCDs (blocks are dispatched round-robin
 // over them) each stream one contiguous eighth of the state.
-template <bool XCD>
+template <int R>
 __device__ __forceinline__ uint64_t block_id() {
-    if (XCD && (gridDim.x & 7) == 0) return uint64_t(blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+    // R = 0 or 1: launch order. R = 8: block b -> region b % 8 (the XCD it
+    // is dispatched to), so each XCD walks one contiguous eighth. R > 8:
+    // R contiguous regions, R / 8 per XCD, visited in parallel.
+    if (R > 1 && (gridDim.x % R) == 0) return uint64_t(blockIdx.x % R) * (gridDim.x / R) + (blockIdx.x / R);
     return blockIdx.x;
 }
 

@@ -136,7 +136,7 @@ __global__ __launch_bounds__(256) void k_dense_mfma(cv<T>* __restrict__ psi, uin
     cv<T> pre[EPT];
     // with xcd, the blocks of one XCD (dispatched round-robin over 8) take
     // consecutive tiles
-    uint64_t tile = xcd ? block_id<true>() : blockIdx.x, base = 0;
+    uint64_t tile = xcd ? block_id<8>() : blockIdx.x, base = 0;
     if (tile < ntiles) {
         base = tile_base(tile);
 #pragma unroll

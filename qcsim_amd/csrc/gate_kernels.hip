@@ -36,7 +36,7 @@ static unsigned grid_for(uint64_t norb, int per_block) {
 // 1-qubit update on the orbit {i0, i0|bit} (plus controls in cmask).
 // MODE 0: generic 2x2 (K3, K8, K13); 1: antidiagonal (K2, K7, K12);
 // 2: diagonal with both entries (K1, K6, K11).
-template <typename T, int MODE, int P, bool NT, bool XCD>
+template <typename T, int MODE, int P, bool NT, int XCD>
 __global__ __launch_bounds__(kThreads) void k_1q(cv<T>* __restrict__ psi, uint64_t norb, Ins ins,
                                                  uint64_t cmask, uint64_t bit, MatD<T, 2> m) {
     const uint64_t base = block_id<XCD>() * (kThreads * P) + threadIdx.x;
@@ -75,7 +75,7 @@ __global__ __launch_bounds__(kThreads) void k_1q(cv<T>* __restrict__ psi, uint64
 
 // Phase on the single amplitude of each orbit whose listed bits are all 1:
 // P/S/T/Z (m00 == 1), CZ/CP/CRz-with-unit-m22 (K6), CCZ (K11).
-template <typename T, int P, bool NT, bool XCD>
+template <typename T, int P, bool NT, int XCD>
 __global__ __launch_bounds__(kThreads) void k_phase(cv<T>* __restrict__ psi, uint64_t norb, Ins ins,
                                                     uint64_t cmask, T pr, T pi) {
     const uint64_t base = block_id<XCD>() * (kThreads * P) + threadIdx.x;
@@ -96,7 +96,7 @@ __global__ __launch_bounds__(kThreads) void k_phase(cv<T>* __restrict__ psi, uin
 
 // Exchange psi[i|ba] <-> psi[i|bb], optionally times a phase (SWAP K4,
 // iSWAP/iSWAP-dagger K5, Fredkin K10 with its control in cmask).
-template <typename T, bool PHASED, int P, bool NT, bool XCD>
+template <typename T, bool PHASED, int P, bool NT, int XCD>
 __global__ __launch_bounds__(kThreads) void k_swap(cv<T>* __restrict__ psi, uint64_t norb, Ins ins,
                                                    uint64_t cmask, uint64_t ba, uint64_t bb, T pr,
                                                    T pi) {
@@ -229,17 +229,21 @@ static unsigned stream_grid(uint64_t A) {
     return unsigned(g);
 }
 
-// Variant dispatch: (P, NT, XCD) -> template instance.
+// Variant dispatch: (P, NT, block regions) -> template instance; v.xcd 0:
+// launch order, 1: 8 regions (one per XCD), 2: 64 regions.
 #define QCS_VARIANT_SWITCH(v, LAUNCH)                                    \
     do {                                                                 \
-        if (v.p == 1 && !v.nt && !v.xcd) LAUNCH(1, false, false);        \
-        else if (v.p == 1 && v.nt && !v.xcd) LAUNCH(1, true, false);     \
-        else if (v.p == 1 && v.nt && v.xcd) LAUNCH(1, true, true);       \
-        else if (v.p == 2 && !v.nt && v.xcd) LAUNCH(2, false, true);     \
-        else if (v.p == 2 && v.nt && !v.xcd) LAUNCH(2, true, false);     \
-        else if (v.p == 4 && !v.nt && v.xcd) LAUNCH(4, false, true);     \
-        else if (v.p == 4 && v.nt && v.xcd) LAUNCH(4, true, true);       \
-        else LAUNCH(2, true, true);                                      \
+        if (v.p == 1 && !v.nt && !v.xcd) LAUNCH(1, false, 0);            \
+        else if (v.p == 1 && v.nt && !v.xcd) LAUNCH(1, true, 0);         \
+        else if (v.p == 1 && v.nt && v.xcd == 1) LAUNCH(1, true, 8);     \
+        else if (v.p == 2 && !v.nt && v.xcd == 1) LAUNCH(2, false, 8);   \
+        else if (v.p == 2 && v.nt && !v.xcd) LAUNCH(2, true, 0);         \
+        else if (v.p == 4 && !v.nt && v.xcd == 1) LAUNCH(4, false, 8);   \
+        else if (v.p == 4 && v.nt && v.xcd == 1) LAUNCH(4, true, 8);     \
+        else if (v.p == 1 && v.nt && v.xcd == 2) LAUNCH(1, true, 64);    \
+        else if (v.p == 2 && v.nt && v.xcd == 2) LAUNCH(2, true, 64);    \
+        else if (v.p == 2 && !v.nt && v.xcd == 2) LAUNCH(2, false, 64);  \
+        else LAUNCH(2, true, 8);                                         \
     } while (0)
 
 template <typename T>
