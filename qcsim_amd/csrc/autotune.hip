@@ -18,10 +18,12 @@
 
 namespace qcs {
 
-// {orbits per thread, nontemporal, block regions (0: none, 1: 8, 2: 64)};
-// ids are persisted in tuning caches, so new variants go at the end.
-static const Variant kVariants[] = {{2, 1, 1}, {2, 0, 1}, {1, 0, 0}, {1, 1, 0}, {1, 1, 1}, {2, 1, 0},
-                                    {4, 0, 1}, {4, 1, 1}, {1, 1, 2}, {2, 1, 2}, {2, 0, 2}};
+// {orbits per thread, nontemporal (0: no, 1: loads and stores, 2: loads,
+// 3: stores), block regions (0: none, 1: 8, 2: 64)}; ids are persisted in
+// tuning caches, so new variants go at the end.
+static const Variant kVariants[] = {{2, 1, 1}, {2, 0, 1}, {1, 0, 0}, {1, 1, 0}, {1, 1, 1}, {2, 1, 0}, {4, 0, 1},
+                                    {4, 1, 1}, {1, 1, 2}, {2, 1, 2}, {2, 0, 2}, {1, 2, 1}, {1, 3, 1}, {1, 3, 0},
+                                    {2, 3, 1}};
 constexpr int kNumVariants = int(sizeof(kVariants) / sizeof(kVariants[0]));
 constexpr int kMaxN = 40;
 

@@ -36,7 +36,7 @@ static unsigned grid_for(uint64_t norb, int per_block) {
 // 1-qubit update on the orbit {i0, i0|bit} (plus controls in cmask).
 // MODE 0: generic 2x2 (K3, K8, K13); 1: antidiagonal (K2, K7, K12);
 // 2: diagonal with both entries (K1, K6, K11).
-template <typename T, int MODE, int P, bool NT, int XCD>
+template <typename T, int MODE, int P, int NT, int XCD>
 __global__ __launch_bounds__(kThreads) void k_1q(cv<T>* __restrict__ psi, uint64_t norb, Ins ins,
                                                  uint64_t cmask, uint64_t bit, MatD<T, 2> m) {
     const uint64_t base = block_id<XCD>() * (kThreads * P) + threadIdx.x;
@@ -47,8 +47,8 @@ __global__ __launch_bounds__(kThreads) void k_1q(cv<T>* __restrict__ psi, uint64
         const uint64_t o = base + uint64_t(j) * kThreads;
         i0[j] = insert_zeros(o, ins) | cmask;
         if (o < norb) {
-            a[j] = ldamp<T, NT>(psi + i0[j]);
-            b[j] = ldamp<T, NT>(psi + (i0[j] | bit));
+            a[j] = ldamp<T, NT == 1 || NT == 2>(psi + i0[j]);
+            b[j] = ldamp<T, NT == 1 || NT == 2>(psi + (i0[j] | bit));
         }
     }
 #pragma unroll
@@ -68,14 +68,14 @@ __global__ __launch_bounds__(kThreads) void k_1q(cv<T>* __restrict__ psi, uint64
             u = cmul<T>(m.re[0], m.im[0], a[j]);
             v = cmul<T>(m.re[3], m.im[3], b[j]);
         }
-        stamp<T, NT>(psi + i0[j], u);
-        stamp<T, NT>(psi + (i0[j] | bit), v);
+        stamp<T, NT == 1 || NT == 3>(psi + i0[j], u);
+        stamp<T, NT == 1 || NT == 3>(psi + (i0[j] | bit), v);
     }
 }
 
 // Phase on the single amplitude of each orbit whose listed bits are all 1:
 // P/S/T/Z (m00 == 1), CZ/CP/CRz-with-unit-m22 (K6), CCZ (K11).
-template <typename T, int P, bool NT, int XCD>
+template <typename T, int P, int NT, int XCD>
 __global__ __launch_bounds__(kThreads) void k_phase(cv<T>* __restrict__ psi, uint64_t norb, Ins ins,
                                                     uint64_t cmask, T pr, T pi) {
     const uint64_t base = block_id<XCD>() * (kThreads * P) + threadIdx.x;
@@ -85,18 +85,18 @@ __global__ __launch_bounds__(kThreads) void k_phase(cv<T>* __restrict__ psi, uin
     for (int j = 0; j < P; ++j) {
         const uint64_t o = base + uint64_t(j) * kThreads;
         i0[j] = insert_zeros(o, ins) | cmask;
-        if (o < norb) a[j] = ldamp<T, NT>(psi + i0[j]);
+        if (o < norb) a[j] = ldamp<T, NT == 1 || NT == 2>(psi + i0[j]);
     }
 #pragma unroll
     for (int j = 0; j < P; ++j) {
         const uint64_t o = base + uint64_t(j) * kThreads;
-        if (o < norb) stamp<T, NT>(psi + i0[j], cmul<T>(pr, pi, a[j]));
+        if (o < norb) stamp<T, NT == 1 || NT == 3>(psi + i0[j], cmul<T>(pr, pi, a[j]));
     }
 }
 
 // Exchange psi[i|ba] <-> psi[i|bb], optionally times a phase (SWAP K4,
 // iSWAP/iSWAP-dagger K5, Fredkin K10 with its control in cmask).
-template <typename T, bool PHASED, int P, bool NT, int XCD>
+template <typename T, bool PHASED, int P, int NT, int XCD>
 __global__ __launch_bounds__(kThreads) void k_swap(cv<T>* __restrict__ psi, uint64_t norb, Ins ins,
                                                    uint64_t cmask, uint64_t ba, uint64_t bb, T pr,
                                                    T pi) {
@@ -108,8 +108,8 @@ __global__ __launch_bounds__(kThreads) void k_swap(cv<T>* __restrict__ psi, uint
         const uint64_t o = base + uint64_t(j) * kThreads;
         i0[j] = insert_zeros(o, ins) | cmask;
         if (o < norb) {
-            a[j] = ldamp<T, NT>(psi + (i0[j] | ba));
-            b[j] = ldamp<T, NT>(psi + (i0[j] | bb));
+            a[j] = ldamp<T, NT == 1 || NT == 2>(psi + (i0[j] | ba));
+            b[j] = ldamp<T, NT == 1 || NT == 2>(psi + (i0[j] | bb));
         }
     }
 #pragma unroll
@@ -117,11 +117,11 @@ __global__ __launch_bounds__(kThreads) void k_swap(cv<T>* __restrict__ psi, uint
         const uint64_t o = base + uint64_t(j) * kThreads;
         if (o >= norb) continue;
         if (PHASED) {
-            stamp<T, NT>(psi + (i0[j] | ba), cmul<T>(pr, pi, b[j]));
-            stamp<T, NT>(psi + (i0[j] | bb), cmul<T>(pr, pi, a[j]));
+            stamp<T, NT == 1 || NT == 3>(psi + (i0[j] | ba), cmul<T>(pr, pi, b[j]));
+            stamp<T, NT == 1 || NT == 3>(psi + (i0[j] | bb), cmul<T>(pr, pi, a[j]));
         } else {
-            stamp<T, NT>(psi + (i0[j] | ba), b[j]);
-            stamp<T, NT>(psi + (i0[j] | bb), a[j]);
+            stamp<T, NT == 1 || NT == 3>(psi + (i0[j] | ba), b[j]);
+            stamp<T, NT == 1 || NT == 3>(psi + (i0[j] | bb), a[j]);
         }
     }
 }
@@ -229,21 +229,26 @@ static unsigned stream_grid(uint64_t A) {
     return unsigned(g);
 }
 
-// Variant dispatch: (P, NT, block regions) -> template instance; v.xcd 0:
-// launch order, 1: 8 regions (one per XCD), 2: 64 regions.
-#define QCS_VARIANT_SWITCH(v, LAUNCH)                                    \
-    do {                                                                 \
-        if (v.p == 1 && !v.nt && !v.xcd) LAUNCH(1, false, 0);            \
-        else if (v.p == 1 && v.nt && !v.xcd) LAUNCH(1, true, 0);         \
-        else if (v.p == 1 && v.nt && v.xcd == 1) LAUNCH(1, true, 8);     \
-        else if (v.p == 2 && !v.nt && v.xcd == 1) LAUNCH(2, false, 8);   \
-        else if (v.p == 2 && v.nt && !v.xcd) LAUNCH(2, true, 0);         \
-        else if (v.p == 4 && !v.nt && v.xcd == 1) LAUNCH(4, false, 8);   \
-        else if (v.p == 4 && v.nt && v.xcd == 1) LAUNCH(4, true, 8);     \
-        else if (v.p == 1 && v.nt && v.xcd == 2) LAUNCH(1, true, 64);    \
-        else if (v.p == 2 && v.nt && v.xcd == 2) LAUNCH(2, true, 64);    \
-        else if (v.p == 2 && !v.nt && v.xcd == 2) LAUNCH(2, false, 64);  \
-        else LAUNCH(2, true, 8);                                         \
+// Variant dispatch: (P, NT, block regions) -> template instance. v.nt 0:
+// plain access, 1: nontemporal loads and stores, 2: loads only, 3: stores
+// only; v.xcd 0: launch order, 1: 8 regions (one per XCD), 2: 64 regions.
+#define QCS_VARIANT_SWITCH(v, LAUNCH)                                         \
+    do {                                                                      \
+        if (v.p == 1 && v.nt == 0 && v.xcd == 0) LAUNCH(1, 0, 0);             \
+        else if (v.p == 1 && v.nt == 1 && v.xcd == 0) LAUNCH(1, 1, 0);        \
+        else if (v.p == 1 && v.nt == 1 && v.xcd == 1) LAUNCH(1, 1, 8);        \
+        else if (v.p == 2 && v.nt == 0 && v.xcd == 1) LAUNCH(2, 0, 8);        \
+        else if (v.p == 2 && v.nt == 1 && v.xcd == 0) LAUNCH(2, 1, 0);        \
+        else if (v.p == 4 && v.nt == 0 && v.xcd == 1) LAUNCH(4, 0, 8);        \
+        else if (v.p == 4 && v.nt == 1 && v.xcd == 1) LAUNCH(4, 1, 8);        \
+        else if (v.p == 1 && v.nt == 1 && v.xcd == 2) LAUNCH(1, 1, 64);       \
+        else if (v.p == 2 && v.nt == 1 && v.xcd == 2) LAUNCH(2, 1, 64);       \
+        else if (v.p == 2 && v.nt == 0 && v.xcd == 2) LAUNCH(2, 0, 64);       \
+        else if (v.p == 1 && v.nt == 2 && v.xcd == 1) LAUNCH(1, 2, 8);        \
+        else if (v.p == 1 && v.nt == 3 && v.xcd == 1) LAUNCH(1, 3, 8);        \
+        else if (v.p == 1 && v.nt == 3 && v.xcd == 0) LAUNCH(1, 3, 0);        \
+        else if (v.p == 2 && v.nt == 3 && v.xcd == 1) LAUNCH(2, 3, 8);        \
+        else LAUNCH(2, 1, 8);                                                 \
     } while (0)
 
 template <typename T>
