@@ -84,6 +84,15 @@ __device__ __forceinline__ void stamp(cv<T>* p, cv<T> a) {
 
 // Block index remap so that the 8 XCDs (blocks are dispatched round-robin
 // over them) each stream one contiguous eighth of the state.
+// Nontemporal loads in the LDS-staged passes (QFT, dense blocks, bit
+// reversal); their stores are always nontemporal. Measured on MI355X at 30
+// qubits: plain loads are slower (QFT 44.2 against 41.8 ms, fused 4-qubit
+// blocks 4.6 % slower).
+#ifndef QCS_STAGED_NT_LOADS
+#define QCS_STAGED_NT_LOADS 1
+#endif
+constexpr bool kStagedNtLoads = QCS_STAGED_NT_LOADS != 0;
+
 template <int R>
 __device__ __forceinline__ uint64_t block_id() {
     // R = 0 or 1: launch order. R = 8: block b -> region b % 8 (the XCD it

@@ -140,7 +140,7 @@ __global__ __launch_bounds__(256) void k_dense_mfma(cv<T>* __restrict__ psi, uin
     if (tile < ntiles) {
         base = tile_base(tile);
 #pragma unroll
-        for (int j = 0; j < EPT; ++j) pre[j] = ldamp<T, true>(psi + (base | off[j]));
+        for (int j = 0; j < EPT; ++j) pre[j] = ldamp<T, kStagedNtLoads>(psi + (base | off[j]));
     }
     for (; tile < ntiles; tile += gridDim.x) {
         const uint64_t cur = base;
@@ -153,7 +153,7 @@ __global__ __launch_bounds__(256) void k_dense_mfma(cv<T>* __restrict__ psi, uin
         if (next < ntiles) {
             base = tile_base(next);
 #pragma unroll
-            for (int j = 0; j < EPT; ++j) pre[j] = ldamp<T, true>(psi + (base | off[j]));
+            for (int j = 0; j < EPT; ++j) pre[j] = ldamp<T, kStagedNtLoads>(psi + (base | off[j]));
         }
         __syncthreads();
         typename Mfma<T>::acc_t acc[NC][NT];

@@ -171,7 +171,7 @@ __global__ __launch_bounds__(256) void k_qft_group(cv<T>* __restrict__ psi, uint
     const uint64_t stride = 1ull << a.lo;
     cv<T> x[D];
 #pragma unroll
-    for (int c = 0; c < D; ++c) x[c] = ldamp<T, true>(p0 + c * stride);
+    for (int c = 0; c < D; ++c) x[c] = ldamp<T, kStagedNtLoads>(p0 + c * stride);
     if (!INV) {
 #pragma unroll
         for (int tl = G - 1; tl >= 0; --tl) {
@@ -250,7 +250,7 @@ __global__ __launch_bounds__(256) void k_qft_group5(cv<T>* __restrict__ psi, uin
     const uint64_t stride = 1ull << a.lo;
     cv<T> x[16];
 #pragma unroll
-    for (int c = 0; c < 16; ++c) x[c] = ldamp<T, true>(p0 + c * stride);
+    for (int c = 0; c < 16; ++c) x[c] = ldamp<T, kStagedNtLoads>(p0 + c * stride);
     if (!INV) {
 #pragma unroll
         for (int tl = kGroupMax - 1; tl >= 0; --tl) {
@@ -341,7 +341,7 @@ __global__ __launch_bounds__(256) void k_qft_lds(cv<T>* __restrict__ psi, uint32
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
             const uint32_t i = threadIdx.x + 256u * k;
-            if (i < C) pre[k] = ldamp<T, true>(psi + (ch << clog) + i);
+            if (i < C) pre[k] = ldamp<T, kStagedNtLoads>(psi + (ch << clog) + i);
         }
     };
     if (chunk < nchunks) load(chunk);
@@ -440,8 +440,8 @@ __global__ __launch_bounds__(256) void k_bitrev_tile(cv<T>* __restrict__ psi, ui
             const uint32_t e = threadIdx.x + 256u * k;
             if (e < E) {
                 const uint64_t off = (uint64_t(e >> a.rowlog) << hi_lo) | (e & (R - 1));
-                p0[k] = ldamp<T, true>(psi + (b0 | off));
-                if (mid_r != mid) p1[k] = ldamp<T, true>(psi + (b1 | off));
+                p0[k] = ldamp<T, kStagedNtLoads>(psi + (b0 | off));
+                if (mid_r != mid) p1[k] = ldamp<T, kStagedNtLoads>(psi + (b1 | off));
             }
         }
     };
