@@ -438,3 +438,28 @@ def test_fused_matches_unfused_long_circuit():
     for k in ("fused4", "fused5"):
         assert out[k]["max_abs_vs_unfused"] < 1e-12, out[k]
         assert out[k]["infidelity_vs_unfused"] < 1e-12, out[k]
+
+
+def test_config1_circuit_against_the_oracle_24_qubits():
+    """The config 1 circuit (60 layers: Haar U(2) on every qubit, then CNOT /
+    CZ / controlled-Haar-U(2) on a random pairing) at 24 qubits, complex128,
+    unfused and fused to widths 4 and 5 (MFMA blocks), against the oracle's
+    gate-by-gate run from |0...0>: max-abs <= 1e-12, infidelity printed."""
+    import bench
+
+    n = 24
+    seq = bench.random_circuit_c1(n, 60)
+    o = OracleRegister(n)
+    for gate, qs in seq:
+        o.apply_gate(gate, *qs)
+    ref = o.state()
+    o.close()
+    for width in (0, 4, 5):
+        g = QubitRegister(n)
+        g.apply_circuit(seq, fuse_width=width)
+        psi = g.getRegisterStorage()
+        err = float(np.max(np.abs(psi - ref)))
+        infid = max(0.0, 1 - abs(np.vdot(ref, psi)) ** 2)
+        print(f"width {width}: max-abs {err:.3e}, infidelity {infid:.3e}")
+        assert err < 1e-12, (width, err)
+        g.close()
