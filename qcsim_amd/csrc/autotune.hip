@@ -19,11 +19,14 @@
 namespace qcs {
 
 // {orbits per thread, nontemporal (0: no, 1: loads and stores, 2: loads,
-// 3: stores), block regions (0: none, 1: 8, 2: 64)}; ids are persisted in
-// tuning caches, so new variants go at the end.
-static const Variant kVariants[] = {{2, 1, 1}, {2, 0, 1}, {1, 0, 0}, {1, 1, 0}, {1, 1, 1}, {2, 1, 0}, {4, 0, 1},
-                                    {4, 1, 1}, {1, 1, 2}, {2, 1, 2}, {2, 0, 2}, {1, 2, 1}, {1, 3, 1}, {1, 3, 0},
-                                    {2, 3, 1}};
+// 3: stores), block regions (0: none, 1: 8, 2: 64), log2 distance of a
+// thread's orbits (k_1q; 0: adjacent chunks)}; ids are persisted in tuning
+// caches, so new variants go at the end.
+static const Variant kVariants[] = {{2, 1, 1}, {2, 0, 1}, {1, 0, 0}, {1, 1, 0}, {1, 1, 1}, {2, 1, 0},
+                                    {4, 0, 1}, {4, 1, 1}, {1, 1, 2}, {2, 1, 2}, {2, 0, 2}, {1, 2, 1},
+                                    {1, 3, 1}, {1, 3, 0}, {2, 3, 1}, {2, 1, 1, 12}, {2, 1, 1, 16},
+                                    {2, 1, 1, 20}, {4, 1, 1, 16}, {2, 3, 1, 16}, {2, 1, 1, 14},
+                                    {2, 1, 1, 18}, {4, 1, 1, 12}, {2, 1, 2, 16}, {4, 1, 1, 14}};
 constexpr int kNumVariants = int(sizeof(kVariants) / sizeof(kVariants[0]));
 constexpr int kMaxN = 40;
 

@@ -169,6 +169,7 @@ int stage_upload(qcs_register* r, const void* src, size_t bytes, const void** de
 // measured default when untuned.
 struct Variant {
     uint8_t p, nt, xcd;
+    uint8_t sp = 0;  // k_1q only: log2 distance of a thread's orbits (0: kThreads)
 };
 Variant pick_variant(const qcs_register* r, uint32_t lowbit);
 int variant_count();

@@ -36,15 +36,28 @@ static unsigned grid_for(uint64_t norb, int per_block) {
 // 1-qubit update on the orbit {i0, i0|bit} (plus controls in cmask).
 // MODE 0: generic 2x2 (K3, K8, K13); 1: antidiagonal (K2, K7, K12);
 // 2: diagonal with both entries (K1, K6, K11).
+// sh = 0: a thread's P orbits are kThreads apart (one contiguous range per
+// block); sh >= 8: they are 2^sh apart, so each block works on P ranges
+// that far from each other at once.
 template <typename T, int MODE, int P, int NT, int XCD>
 __global__ __launch_bounds__(kThreads) void k_1q(cv<T>* __restrict__ psi, uint64_t norb, Ins ins,
-                                                 uint64_t cmask, uint64_t bit, MatD<T, 2> m) {
-    const uint64_t base = block_id<XCD>() * (kThreads * P) + threadIdx.x;
+                                                 uint64_t cmask, uint64_t bit, MatD<T, 2> m, uint32_t sh) {
+    const uint64_t bb = block_id<XCD>();
+    uint64_t base, step;
+    if (sh) {
+        const uint32_t lb = sh - 8;  // blocks per range of 2^sh orbits
+        base = ((bb >> lb) << (sh + (P == 4 ? 2 : P == 2 ? 1 : 0))) + ((bb & ((1ull << lb) - 1)) << 8) +
+               threadIdx.x;
+        step = 1ull << sh;
+    } else {
+        base = bb * (kThreads * P) + threadIdx.x;
+        step = kThreads;
+    }
     cv<T> a[P], b[P];
     uint64_t i0[P];
 #pragma unroll
     for (int j = 0; j < P; ++j) {
-        const uint64_t o = base + uint64_t(j) * kThreads;
+        const uint64_t o = base + uint64_t(j) * step;
         i0[j] = insert_zeros(o, ins) | cmask;
         if (o < norb) {
             a[j] = ldamp<T, NT == 1 || NT == 2>(psi + i0[j]);
@@ -53,7 +66,7 @@ __global__ __launch_bounds__(kThreads) void k_1q(cv<T>* __restrict__ psi, uint64
     }
 #pragma unroll
     for (int j = 0; j < P; ++j) {
-        const uint64_t o = base + uint64_t(j) * kThreads;
+        const uint64_t o = base + uint64_t(j) * step;
         if (o >= norb) continue;
         cv<T> u, v;
         if (MODE == 0) {
@@ -273,18 +286,21 @@ static int k1q_t(qcs_register* r, int mode, uint32_t target, const uint32_t* ctr
     if (v.p == 0) v = pick_variant(r, low);
     auto* psi = static_cast<cv<T>*>(r->psi);
     const uint64_t bit = 1ull << target;
+    // spread layout only when every block gets v.p full ranges of 2^sh orbits
+    uint32_t sh = v.sp;
+    if (sh && (v.p < 2 || sh < 8 || norb < (uint64_t(v.p) << sh))) sh = 0;
 #define QCS_L1Q(PP, NTT, XX)                                                                                   \
     do {                                                                                                       \
         const unsigned g = grid_for(norb, kThreads * PP);                                                      \
         if (mode == 0)                                                                                         \
             hipLaunchKernelGGL((k_1q<T, 0, PP, NTT, XX>), dim3(g), dim3(kThreads), 0, r->stream, psi, norb, ins, \
-                               cmask, bit, mm);                                                                \
+                               cmask, bit, mm, sh);                                                            \
         else if (mode == 1)                                                                                    \
             hipLaunchKernelGGL((k_1q<T, 1, PP, NTT, XX>), dim3(g), dim3(kThreads), 0, r->stream, psi, norb, ins, \
-                               cmask, bit, mm);                                                                \
+                               cmask, bit, mm, sh);                                                            \
         else                                                                                                   \
             hipLaunchKernelGGL((k_1q<T, 2, PP, NTT, XX>), dim3(g), dim3(kThreads), 0, r->stream, psi, norb, ins, \
-                               cmask, bit, mm);                                                                \
+                               cmask, bit, mm, sh);                                                            \
     } while (0)
     QCS_VARIANT_SWITCH(v, QCS_L1Q);
 #undef QCS_L1Q
