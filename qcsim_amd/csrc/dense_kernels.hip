@@ -96,7 +96,11 @@ __global__ __launch_bounds__(256) void k_dense_mfma(cv<T>* __restrict__ psi, uin
     // 64 g + 16 w .. 64 g + 16 w + 15 of every group g
     constexpr int D = 1 << K, RD = 2 * D, NB = 64 * NC, E = D * NB, EPT = E / 256, LDX = NB + 2;
     constexpr int NT = RD / 16, NS = RD / 4;
+    // sX holds a tile's inputs, sY its results: a block synchronises twice
+    // per tile (inputs written; results written), since results never
+    // overwrite inputs that another wave may still be reading.
     __shared__ __attribute__((aligned(16))) T sX[RD * LDX];
+    __shared__ __attribute__((aligned(16))) T sY[RD * LDX];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     // per-thread element map (the same for every tile)
     uint64_t off[EPT];
@@ -169,23 +173,24 @@ __global__ __launch_bounds__(256) void k_dense_mfma(cv<T>* __restrict__ psi, uin
 #pragma unroll
                 for (int t = 0; t < NT; ++t) acc[g][t] = Mfma<T>::step(af[t][s], b, acc[g][t]);
             }
-        __syncthreads();
 #pragma unroll
         for (int g = 0; g < NC; ++g)
 #pragma unroll
             for (int t = 0; t < NT; ++t)
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
-                    sX[(16 * t + Mfma<T>::row(lane, r)) * LDX + 64 * g + 16 * w + (lane & 15)] = acc[g][t][r];
+                    sY[(16 * t + Mfma<T>::row(lane, r)) * LDX + 64 * g + 16 * w + (lane & 15)] = acc[g][t][r];
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < EPT; ++j) {
             cv<T> v;
-            v.x = sX[row[j] * LDX + col[j]];
-            v.y = sX[(D + row[j]) * LDX + col[j]];
+            v.x = sY[row[j] * LDX + col[j]];
+            v.y = sY[(D + row[j]) * LDX + col[j]];
             stamp<T, true>(psi + (cur | off[j]), v);
         }
-        __syncthreads();
+        // no barrier here: the next tile's inputs go to sX (no wave reads it
+        // any more, all passed the barrier above), and its results reach sY
+        // only after the next barrier, once every wave has stored this one
     }
 }
 
