@@ -215,9 +215,22 @@ static int kdense_mfma(qcs_register* r, const uint32_t* qubits, const double* ma
     TileGeom tg{};
     uint64_t gmask = 0;
     for (int i = 0; i < K; ++i) gmask |= 1ull << qubits[i];
+    // Orbit bits of a tile: the lowest OB bits outside the block, except
+    // that when those make the tile's run from bit 0 between 128 and 1024
+    // amplitudes long, the last one moves to the first free bit at or above
+    // 12 (the tile then covers two ranges that far apart). Measured on
+    // MI355X at 30 qubits: 6-15 % faster for such tiles, slower for runs of
+    // 64 or 2048 amplitudes.
     uint32_t orb[OB], no = 0;
     for (uint32_t b = 0; no < uint32_t(OB); ++b)
         if (!(gmask >> b & 1)) orb[no++] = b;
+    {
+        uint32_t run = 0;  // log2 of the contiguous run from bit 0
+        while (run < 64 && ((gmask >> run & 1) || std::find(orb, orb + OB, run) != orb + OB)) ++run;
+        uint32_t b = std::max(12u, orb[OB - 1] + 1);
+        while (b < r->n && (gmask >> b & 1)) ++b;
+        if (run >= 7 && run <= 10 && b < r->n) orb[OB - 1] = b;
+    }
     uint32_t all[K + OB];
     for (int i = 0; i < K; ++i) all[i] = qubits[i];
     for (int j = 0; j < OB; ++j) all[K + j] = orb[j];
