@@ -93,3 +93,36 @@ def test_layered_circuit_fuses_and_keeps_single_gates():
     # a lone CZ stays a CZ (keeps its quarter-traffic kernel)
     out1, n1 = fuse(to_struct([(G.ControlledZGate(), [2, 7])]), 5)
     assert n1 == 1 and out1[0].flags == 0 and out1[0].traits == G.ControlledZGate.traits
+
+
+def test_layered_30_qubit_circuit_block_count():
+    """60 layers on 30 qubits (a Haar U(2) on every qubit, then CNOT / CZ on
+    a random pairing): the greedy block growth takes 215 blocks of <= 5
+    qubits with this seed (gate-order growth took about 380); the bound leaves a
+    little room and catches a pass that stops taking gates past the first
+    blocked one."""
+    n, layers = 30, 60
+    rng = np.random.default_rng(11)
+    seq = []
+    for _ in range(layers):
+        for q in range(n):
+            seq.append((G.SingleQubitGate(G.haar_u2(rng)), [q]))
+        perm = rng.permutation(n)
+        for a, b in zip(perm[0::2], perm[1::2]):
+            gate = G.CNOTGate() if rng.integers(2) else G.ControlledZGate()
+            seq.append((gate, [int(a), int(b)]))
+    out, nout = fuse(to_struct(seq), 5)
+    assert nout <= 230
+    assert all(out[i].k <= 5 for i in range(nout))
+    # the blocks of a slice (the gates on qubits 0..7) reproduce it exactly
+    small = [(g, qs) for g, qs in seq[:200] if max(qs) < 8]
+    psi = random_state(rng, 8)
+    ref, fr = OracleRegister(8), OracleRegister(8)
+    ref.set_state(psi)
+    fr.set_state(psi)
+    for gate, qs in small:
+        ref.apply_gate(gate, *qs)
+    o2, n2 = fuse(to_struct(small), 5)
+    for i in range(n2):
+        apply_struct(fr, o2[i])
+    assert np.max(np.abs(fr.state() - ref.state())) < 1e-12
