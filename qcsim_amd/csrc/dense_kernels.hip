@@ -4,12 +4,12 @@
 // only in the block's k qubits (local bit i <-> qubits[i]); this is the
 // Kronecker embedding of SimpleGates.h:211-232/311-333/411-434 generalised
 // to k bits. k <= 3 uses the gate kernels. k = 4, 5 run on the matrix cores
-// (k_dense_mfma): the complex block M becomes the real 2D x 2D matrix
-// R = [[Re M, -Im M], [Im M, Re M]] acting on [Re x; Im x]; a workgroup
-// stages 64 orbits (2D x 64 reals) in LDS and each wave multiplies 16 of
-// them with v_mfma_f64_16x16x4f64 (complex128) or v_mfma_f32_16x16x4f32
-// (complex64), the R fragments held in registers. 8 * 2^k FLOP per
-// amplitude. k_dense (matrix in LDS, orbit in registers, no MFMA) is the
+// (k_dense_mfma): a workgroup stages 64 orbits ([Re x; Im x] as 2D x 64
+// reals) in LDS and each wave multiplies 16 of them with
+// v_mfma_f64_16x16x4f64 (complex128) or v_mfma_f32_16x16x4f32 (complex64),
+// as three real D x D products (Gauss's product of complex numbers, applied
+// to matrices), the factors' fragments held in registers. 8 * 2^k FLOP per
+// amplitude for the complex product; the MFMA units execute 6 * 2^k. k_dense (matrix in LDS, orbit in registers, no MFMA) is the
 // fallback for states too small to tile.
 #include <algorithm>
 #include <cstdlib>
@@ -95,7 +95,7 @@ __global__ __launch_bounds__(256) void k_dense_mfma(cv<T>* __restrict__ psi, uin
     // NC groups of 64 orbit columns per tile; wave w multiplies columns
     // 64 g + 16 w .. 64 g + 16 w + 15 of every group g
     constexpr int D = 1 << K, RD = 2 * D, NB = 64 * NC, E = D * NB, EPT = E / 256, LDX = NB + 2;
-    constexpr int NT = RD / 16, NS = RD / 4;
+    constexpr int NT = D / 16, NS = D / 4;
     // sX holds a tile's inputs, sY its results: a block synchronises twice
     // per tile (inputs written; results written), since results never
     // overwrite inputs that another wave may still be reading.
@@ -121,12 +121,16 @@ __global__ __launch_bounds__(256) void k_dense_mfma(cv<T>* __restrict__ psi, uin
         row[j] = c;
         col[j] = ol;
     }
-    // R fragments: A[t][s] = R[16t + lane%16][4s + lane/16]
-    T af[NT][NS];
+    // Fragments of the three D x D factors (Re M, Im M, Re M + Im M):
+    // af[m][t][s] = factor m at row 16 t + lane % 16, column 4 s + lane / 16
+    T af[3][NT][NS];
 #pragma unroll
-    for (int t = 0; t < NT; ++t)
+    for (int m = 0; m < 3; ++m)
 #pragma unroll
-        for (int s = 0; s < NS; ++s) af[t][s] = R[(16 * t + (lane & 15)) * RD + 4 * s + (lane >> 4)];
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+                af[m][t][s] = R[(m * D + 16 * t + (lane & 15)) * D + 4 * s + (lane >> 4)];
     // The next tile's amplitudes are loaded into registers while the
     // current one is multiplied (tiles are disjoint and owned by one block).
     auto tile_base = [&](uint64_t tile) {
@@ -160,26 +164,38 @@ __global__ __launch_bounds__(256) void k_dense_mfma(cv<T>* __restrict__ psi, uin
             for (int j = 0; j < EPT; ++j) pre[j] = ldamp<T, kStagedNtLoads>(psi + (base | off[j]));
         }
         __syncthreads();
-        typename Mfma<T>::acc_t acc[NC][NT];
+        // Three real products per complex one: with M = A + iB and
+        // x = X + iY, P1 = A X, P2 = B Y, P3 = (A + B)(X + Y); then
+        // Re Mx = P1 - P2 and Im Mx = P3 - P1 - P2.
+        using acc_t = typename Mfma<T>::acc_t;
+        acc_t p1[NC][NT], p2[NC][NT], p3[NC][NT];
 #pragma unroll
         for (int g = 0; g < NC; ++g)
 #pragma unroll
-            for (int t = 0; t < NT; ++t) acc[g][t] = typename Mfma<T>::acc_t{0, 0, 0, 0};
+            for (int t = 0; t < NT; ++t) p1[g][t] = p2[g][t] = p3[g][t] = acc_t{0, 0, 0, 0};
 #pragma unroll
         for (int g = 0; g < NC; ++g)
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
-                const T b = sX[(4 * s + (lane >> 4)) * LDX + 64 * g + 16 * w + (lane & 15)];
+                const int c = 64 * g + 16 * w + (lane & 15), rr = 4 * s + (lane >> 4);
+                const T bx = sX[rr * LDX + c], by = sX[(D + rr) * LDX + c], bs = bx + by;
 #pragma unroll
-                for (int t = 0; t < NT; ++t) acc[g][t] = Mfma<T>::step(af[t][s], b, acc[g][t]);
+                for (int t = 0; t < NT; ++t) {
+                    p1[g][t] = Mfma<T>::step(af[0][t][s], bx, p1[g][t]);
+                    p2[g][t] = Mfma<T>::step(af[1][t][s], by, p2[g][t]);
+                    p3[g][t] = Mfma<T>::step(af[2][t][s], bs, p3[g][t]);
+                }
             }
 #pragma unroll
         for (int g = 0; g < NC; ++g)
 #pragma unroll
             for (int t = 0; t < NT; ++t)
 #pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    sY[(16 * t + Mfma<T>::row(lane, r)) * LDX + 64 * g + 16 * w + (lane & 15)] = acc[g][t][r];
+                for (int r = 0; r < 4; ++r) {
+                    const int rw = 16 * t + Mfma<T>::row(lane, r), c = 64 * g + 16 * w + (lane & 15);
+                    sY[rw * LDX + c] = p1[g][t][r] - p2[g][t][r];
+                    sY[(D + rw) * LDX + c] = p3[g][t][r] - p1[g][t][r] - p2[g][t][r];
+                }
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < EPT; ++j) {
@@ -200,15 +216,15 @@ static int kdense_mfma(qcs_register* r, const uint32_t* qubits, const double* ma
     // for 4-qubit blocks at 30 qubits (6.77 against 6.6-6.9 ms per launch)
     // at lower occupancy, and do not fit the register budget of 5-qubit ones.
     constexpr int NC = 1, OB = 6;
-    constexpr int D = 1 << K, RD = 2 * D;
-    std::vector<T> R(size_t(RD) * RD);
+    constexpr int D = 1 << K;
+    // factors Re M, Im M and Re M + Im M, stacked (3 D x D, row-major)
+    std::vector<T> R(size_t(3) * D * D);
     for (int i = 0; i < D; ++i)
         for (int j = 0; j < D; ++j) {
-            const T re = T(mat_host[2 * (i * D + j)]), im = T(mat_host[2 * (i * D + j) + 1]);
-            R[size_t(i) * RD + j] = re;
-            R[size_t(i) * RD + D + j] = -im;
-            R[size_t(D + i) * RD + j] = im;
-            R[size_t(D + i) * RD + D + j] = re;
+            const double re = mat_host[2 * (i * D + j)], im = mat_host[2 * (i * D + j) + 1];
+            R[size_t(i) * D + j] = T(re);
+            R[size_t(D + i) * D + j] = T(im);
+            R[size_t(2 * D + i) * D + j] = T(re + im);
         }
     const void* dmat = nullptr;
     if (const int e = stage_upload(r, R.data(), sizeof(T) * R.size(), &dmat)) return e;
