@@ -8,9 +8,12 @@
 // the fastest in a process-wide table keyed by (precision, n, lowest qubit).
 // Timing uses the generic 1-qubit kernel with the identity matrix, which
 // moves exactly the bytes of a real gate and leaves every finite amplitude
-// unchanged in value. Untuned entries fall back to the default measured on MI355X
-// at 30 qubits: plain loads for qubits 0-2, nontemporal above, both with two
-// orbits per thread and XCD-ordered blocks.
+// unchanged in value. Untuned entries fall back to the defaults measured on
+// MI355X: for complex128 k_1q one amplitude per lane (k_1q_split) with
+// nontemporal access and XCD-ordered blocks, the fastest layout for 27 of
+// the 30 targets at 30 qubits; otherwise (complex64, where 8-byte lanes
+// stream slower, and the other kernels) plain loads for qubits 0-2,
+// nontemporal above, both with two orbits per thread and XCD-ordered blocks.
 #include <atomic>
 #include <mutex>
 
@@ -20,13 +23,16 @@ namespace qcs {
 
 // {orbits per thread, nontemporal (0: no, 1: loads and stores, 2: loads,
 // 3: stores), block regions (0: none, 1: 8, 2: 64), log2 distance of a
-// thread's orbits (k_1q; 0: adjacent chunks)}; ids are persisted in tuning
-// caches, so new variants go at the end.
+// thread's orbits (k_1q; 0: adjacent chunks), one amplitude per lane (k_1q;
+// k_1q_split)}; ids are persisted in tuning caches, so new variants go at
+// the end.
 static const Variant kVariants[] = {{2, 1, 1}, {2, 0, 1}, {1, 0, 0}, {1, 1, 0}, {1, 1, 1}, {2, 1, 0},
                                     {4, 0, 1}, {4, 1, 1}, {1, 1, 2}, {2, 1, 2}, {2, 0, 2}, {1, 2, 1},
                                     {1, 3, 1}, {1, 3, 0}, {2, 3, 1}, {2, 1, 1, 12}, {2, 1, 1, 16},
                                     {2, 1, 1, 20}, {4, 1, 1, 16}, {2, 3, 1, 16}, {2, 1, 1, 14},
-                                    {2, 1, 1, 18}, {4, 1, 1, 12}, {2, 1, 2, 16}, {4, 1, 1, 14}};
+                                    {2, 1, 1, 18}, {4, 1, 1, 12}, {2, 1, 2, 16}, {4, 1, 1, 14},
+                                    {1, 1, 1, 0, 1}, {1, 1, 0, 0, 1}, {1, 0, 1, 0, 1}, {1, 3, 1, 0, 1},
+                                    {1, 1, 2, 0, 1}};
 constexpr int kNumVariants = int(sizeof(kVariants) / sizeof(kVariants[0]));
 constexpr int kMaxN = 40;
 
@@ -47,12 +53,13 @@ void set_tuned(int prec, uint32_t n, uint32_t q, int vid) {
     if (n <= kMaxN && q < kMaxN) g_table[prec][n][q].store(int8_t(vid), std::memory_order_relaxed);
 }
 
-Variant pick_variant(const qcs_register* r, uint32_t lowbit) {
+Variant pick_variant(const qcs_register* r, uint32_t lowbit, bool k1q) {
     std::call_once(g_init, init_table);
     if (r->n <= kMaxN && lowbit < kMaxN) {
         const int vid = g_table[r->prec][r->n][lowbit].load(std::memory_order_relaxed);
         if (vid >= 0) return kVariants[vid];
     }
+    if (k1q && r->prec == QCS_COMPLEX128) return Variant{1, 1, 1, 0, 1};
     return lowbit < 3 ? Variant{2, 0, 1} : Variant{2, 1, 1};
 }
 

@@ -14,6 +14,8 @@
 #include <cfloat>
 #include <vector>
 
+#include <algorithm>
+
 #include "common.h"
 
 namespace qcs {
@@ -152,11 +154,12 @@ __device__ __forceinline__ double chunk_thread_prefix(const cv<T>* __restrict__ 
 
 template <typename T>
 __global__ __launch_bounds__(kScanThreads) void k_chunk_sums(const cv<T>* __restrict__ psi, ChunkGeom g,
-                                                             double* __restrict__ chunk) {
+                                                             double* __restrict__ chunk, uint64_t c0) {
     __shared__ double lds[4];
     double total;
-    chunk_thread_prefix<T>(psi, uint64_t(blockIdx.x) * g.clen, g, lds, &total);
-    if (threadIdx.x == 0) chunk[blockIdx.x] = total;
+    const uint64_t c = c0 + blockIdx.x;
+    chunk_thread_prefix<T>(psi, c * g.clen, g, lds, &total);
+    if (threadIdx.x == 0) chunk[c] = total;
 }
 
 // In-place inclusive scan of chunk totals by one block of 1024 threads.
@@ -390,10 +393,16 @@ static int cdf_search_t(qcs_register* r, uint64_t nshots, const double* probs_ho
     const ChunkGeom g = chunk_geom(r);
     const uint64_t nchunks = r->A / g.clen;
     const auto* psi = static_cast<const cv<T>*>(r->psi);
-    hipLaunchKernelGGL((k_chunk_sums<T>), dim3(unsigned(nchunks)), dim3(kScanThreads), 0, r->stream, psi, g,
-                       r->chunk);
-    int e = check_launch(r, "k_chunk_sums");
-    if (e) return e;
+    // at most 2^22 chunks (2^30 work-items) per launch: a grid's size in
+    // work-items must stay below 2^32
+    int e = 0;
+    for (uint64_t c0 = 0; c0 < nchunks; c0 += (1ull << 22)) {
+        const uint64_t cnt = std::min<uint64_t>(1ull << 22, nchunks - c0);
+        hipLaunchKernelGGL((k_chunk_sums<T>), dim3(unsigned(cnt)), dim3(kScanThreads), 0, r->stream, psi, g,
+                           r->chunk, c0);
+        e = check_launch(r, "k_chunk_sums");
+        if (e) return e;
+    }
     hipLaunchKernelGGL(k_scan_chunks, dim3(1), dim3(1024), 0, r->stream, r->chunk, nchunks);
     e = check_launch(r, "k_scan_chunks");
     if (e) return e;

@@ -50,3 +50,51 @@ def test_every_variant_matches_the_oracle(precision, tol):
             lib().qcs_set_tuned_variant(prec, n, q, -1)
     g.close()
     o.close()
+
+
+@pytest.mark.parametrize("precision,tol", [("c128", 1e-12), ("c64", 2e-6)])
+def test_every_variant_every_target_and_control(precision, tol):
+    """Each variant forced in turn, on every target of a 12-qubit state, for
+    the three k_1q modes (generic H / U, antidiagonal X / CNOT / Toffoli,
+    diagonal Rz / CRz) with 0, 1 and 2 controls at random positions below
+    and above the target: the one-amplitude-per-lane layouts take their
+    partner from another lane of the block, in a block of 256 consecutive
+    amplitudes or of two 128-amplitude halves, depending on the target's
+    position among the non-control bits."""
+    n = 12
+    prec = QCS_COMPLEX128 if precision == "c128" else QCS_COMPLEX64
+    rng = np.random.default_rng(12)
+    psi = random_state(rng, n)
+    o = OracleRegister(n)
+    o.set_state(psi)
+    g = QubitRegister(n, precision=precision)
+    g.setRegisterStorageFastNoNormalize(psi.copy())
+    nv = lib().qcs_autotune_variants()
+    try:
+        for v in range(nv):
+            for q in range(n):
+                check(lib().qcs_set_tuned_variant(prec, n, q, v))
+            for t in range(n):
+                others = [q for q in range(n) if q != t]
+                c1, c2 = (int(x) for x in rng.choice(others, size=2, replace=False))
+                th = float(rng.uniform(0, 2 * np.pi))
+                seq = [
+                    (G.HadamardGate(), [t]),
+                    (G.SingleQubitGate(G.haar_u2(rng)), [t]),
+                    (G.PauliXGate(), [t]),
+                    (G.RzGate(th), [t]),
+                    (G.CNOTGate(), [t, c1]),
+                    (G.ControlledUGate(th, 0.3, 1.1, 0.2), [t, c2]),
+                    (G.ControlledRzGate(th), [t, c1]),
+                    (G.ToffoliGate(), [t, c1, c2]),
+                ]
+                for gate, qs in seq:
+                    g.ApplyGate(gate, *qs)
+                    o.apply_gate(gate, *qs)
+            err = np.max(np.abs(g.getRegisterStorage() - o.state()))
+            assert err < tol, (v, err)
+    finally:
+        for q in range(n):
+            lib().qcs_set_tuned_variant(prec, n, q, -1)
+    g.close()
+    o.close()
