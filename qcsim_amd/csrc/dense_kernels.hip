@@ -79,6 +79,24 @@ template <> struct Mfma<float> {
     __device__ static int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
+// 16-byte global accesses: one complex128 or two adjacent complex64
+// amplitudes per lane (MI355X streams 16 B lanes faster than 8 B ones).
+template <typename T> struct V16;
+template <> struct V16<double> { typedef double type __attribute__((ext_vector_type(2))); };
+template <> struct V16<float> { typedef float type __attribute__((ext_vector_type(4))); };
+
+template <typename T>
+__device__ __forceinline__ typename V16<T>::type ld16(const cv<T>* p) {
+    const auto* q = reinterpret_cast<const typename V16<T>::type*>(p);
+    if (kStagedNtLoads) return __builtin_nontemporal_load(q);
+    return *q;
+}
+
+template <typename T>
+__device__ __forceinline__ void st16(cv<T>* p, typename V16<T>::type v) {
+    __builtin_nontemporal_store(v, reinterpret_cast<typename V16<T>::type*>(p));
+}
+
 // Tile geometry: the block's K qubits plus the 6 lowest other bits form the
 // 11 (K=5) / 10 (K=4) "tile positions", ascending. Element e of a tile
 // deposits its bits into those positions, so consecutive lanes touch
@@ -95,6 +113,11 @@ __global__ __launch_bounds__(256) void k_dense_mfma(cv<T>* __restrict__ psi, uin
     // NC groups of 64 orbit columns per tile; wave w multiplies columns
     // 64 g + 16 w .. 64 g + 16 w + 15 of every group g
     constexpr int D = 1 << K, RD = 2 * D, NB = 64 * NC, E = D * NB, EPT = E / 256, LDX = NB + 2;
+    // VP amplitudes per 16-byte access: elements VP e' .. VP e' + VP - 1 of
+    // the tile are adjacent in memory, since tile position 0 is always
+    // qubit 0 (a block qubit or the lowest orbit bit)
+    constexpr int VP = sizeof(T) == 4 ? 2 : 1, NV = EPT / VP;
+    using vt = typename V16<T>::type;
     constexpr int NT = D / 16, NS = D / 4;
     // sX holds a tile's inputs, sY its results: a block synchronises twice
     // per tile (inputs written; results written), since results never
@@ -102,12 +125,15 @@ __global__ __launch_bounds__(256) void k_dense_mfma(cv<T>* __restrict__ psi, uin
     __shared__ __attribute__((aligned(16))) T sX[RD * LDX];
     __shared__ __attribute__((aligned(16))) T sY[RD * LDX];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    // per-thread element map (the same for every tile)
-    uint64_t off[EPT];
-    int row[EPT], col[EPT];
+    // per-thread element map (the same for every tile): the first element
+    // of each access; the second (VP = 2) differs in element bit 0
+    uint64_t off[NV];
+    int row[NV], col[NV];
+    const int rb = (tg.role[0] & 0x80) ? 0 : 1 << tg.role[0];
+    const int cb = (tg.role[0] & 0x80) ? 1 << (tg.role[0] & 0x7f) : 0;
 #pragma unroll
-    for (int j = 0; j < EPT; ++j) {
-        const uint32_t e = uint32_t(tid + 256 * j);
+    for (int j = 0; j < NV; ++j) {
+        const uint32_t e = uint32_t(VP * (tid + 256 * j));
         uint64_t o = 0;
         int c = 0, ol = 0;
         for (uint32_t p = 0; p < tg.npos; ++p) {
@@ -141,27 +167,31 @@ __global__ __launch_bounds__(256) void k_dense_mfma(cv<T>* __restrict__ psi, uin
         }
         return b;
     };
-    cv<T> pre[EPT];
+    vt pre[NV];
     // with xcd, the blocks of one XCD (dispatched round-robin over 8) take
     // consecutive tiles
     uint64_t tile = xcd ? block_id<8>() : blockIdx.x, base = 0;
     if (tile < ntiles) {
         base = tile_base(tile);
 #pragma unroll
-        for (int j = 0; j < EPT; ++j) pre[j] = ldamp<T, kStagedNtLoads>(psi + (base | off[j]));
+        for (int j = 0; j < NV; ++j) pre[j] = ld16<T>(psi + (base | off[j]));
     }
     for (; tile < ntiles; tile += gridDim.x) {
         const uint64_t cur = base;
 #pragma unroll
-        for (int j = 0; j < EPT; ++j) {
-            sX[row[j] * LDX + col[j]] = pre[j].x;
-            sX[(D + row[j]) * LDX + col[j]] = pre[j].y;
+        for (int j = 0; j < NV; ++j) {
+            sX[row[j] * LDX + col[j]] = pre[j][0];
+            sX[(D + row[j]) * LDX + col[j]] = pre[j][1];
+            if constexpr (VP == 2) {
+                sX[(row[j] | rb) * LDX + (col[j] | cb)] = pre[j][2];
+                sX[(D + (row[j] | rb)) * LDX + (col[j] | cb)] = pre[j][3];
+            }
         }
         const uint64_t next = tile + gridDim.x;
         if (next < ntiles) {
             base = tile_base(next);
 #pragma unroll
-            for (int j = 0; j < EPT; ++j) pre[j] = ldamp<T, kStagedNtLoads>(psi + (base | off[j]));
+            for (int j = 0; j < NV; ++j) pre[j] = ld16<T>(psi + (base | off[j]));
         }
         __syncthreads();
         // Three real products per complex one: with M = A + iB and
@@ -198,11 +228,15 @@ __global__ __launch_bounds__(256) void k_dense_mfma(cv<T>* __restrict__ psi, uin
                 }
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < EPT; ++j) {
-            cv<T> v;
-            v.x = sY[row[j] * LDX + col[j]];
-            v.y = sY[(D + row[j]) * LDX + col[j]];
-            stamp<T, true>(psi + (cur | off[j]), v);
+        for (int j = 0; j < NV; ++j) {
+            vt v;
+            v[0] = sY[row[j] * LDX + col[j]];
+            v[1] = sY[(D + row[j]) * LDX + col[j]];
+            if constexpr (VP == 2) {
+                v[2] = sY[(row[j] | rb) * LDX + (col[j] | cb)];
+                v[3] = sY[(D + (row[j] | rb)) * LDX + (col[j] | cb)];
+            }
+            st16<T>(psi + (cur | off[j]), v);
         }
         // no barrier here: the next tile's inputs go to sX (no wave reads it
         // any more, all passed the barrier above), and its results reach sY
@@ -260,6 +294,8 @@ static int kdense_mfma(qcs_register* r, const uint32_t* qubits, const double* ma
         for (int j = 0; j < OB; ++j)
             if (orb[j] == all[p]) tg.role[p] = uint8_t(0x80 | j);
     }
+    // complex64 tiles load amplitude pairs (2v, 2v + 1): position 0 must be bit 0
+    if (tg.pos[0] != 0) return set_error(QCS_EDEVICE, "dense tile geometry without qubit 0");
     const uint64_t ntiles = r->A >> (K + OB);
     const unsigned grid = unsigned(ntiles < 4096 ? ntiles : 4096);
     static const int xcd = [] {
