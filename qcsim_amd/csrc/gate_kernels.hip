@@ -307,6 +307,21 @@ static unsigned stream_grid(uint64_t A) {
     return unsigned(g);
 }
 
+// Gates that touch only the amplitudes with some bit set (controls, phase
+// bits, the a != b half of a swap) use only part of each 128-byte line when
+// that bit is below 3 (complex128) or 4 (complex64). Nontemporal loads then
+// lose the untouched part, and the partial stores cost a read-modify-write:
+// measured on MI355X at 30 qubits (S on qubit 0, CZ/SWAP on 0-1, CNOT with
+// control 1) 8.1-14.7 ms with nontemporal loads against 6.05-6.3 ms with
+// plain loads and nontemporal stores. Such launches take plain loads,
+// nontemporal stores, one orbit per thread and XCD-ordered blocks (the
+// one-amplitude-per-lane choice of k_1q is kept).
+static Variant partial_line_variant(const qcs_register* r, Variant v, uint32_t lowest_condition_bit) {
+    const uint32_t line_bits = r->prec == QCS_COMPLEX128 ? 3 : 4;
+    if (lowest_condition_bit >= line_bits) return v;
+    return Variant{1, 3, 1, 0, v.x};
+}
+
 // Variant dispatch: (P, NT, block regions) -> template instance. v.nt 0:
 // plain access, 1: nontemporal loads and stores, 2: loads only, 3: stores
 // only; v.xcd 0: launch order, 1: 8 regions (one per XCD), 2: 64 regions.
@@ -348,7 +363,12 @@ static int k1q_t(qcs_register* r, int mode, uint32_t target, const uint32_t* ctr
         mm.re[i] = T(m[i].re);
         mm.im[i] = T(m[i].im);
     }
-    if (v.p == 0) v = pick_variant(r, low, true);
+    if (v.p == 0) {
+        v = pick_variant(r, low, true);
+        uint32_t lowc = 64;
+        for (int i = 0; i < nctrl; ++i) lowc = ctrls[i] < lowc ? ctrls[i] : lowc;
+        v = partial_line_variant(r, v, lowc);
+    }
     auto* psi = static_cast<cv<T>*>(r->psi);
     const uint64_t bit = 1ull << target;
     // spread layout only when every block gets v.p full ranges of 2^sh orbits
@@ -422,7 +442,7 @@ static int kphase_t(qcs_register* r, const uint32_t* bits, int nbits, hcplx ph) 
     }
     const Ins ins = make_ins(bits, nbits);
     const uint64_t norb = r->A >> nbits;
-    const Variant v = pick_variant(r, low);
+    const Variant v = partial_line_variant(r, pick_variant(r, low), low);
     auto* psi = static_cast<cv<T>*>(r->psi);
 #define QCS_LPH(PP, NTT, XX)                                                                                   \
     do {                                                                                                       \
@@ -453,7 +473,7 @@ static int kswap_t(qcs_register* r, uint32_t a, uint32_t b, const uint32_t* ctrl
     }
     const Ins ins = make_ins(pos, 2 + nctrl);
     const uint64_t norb = r->A >> (2 + nctrl);
-    const Variant v = pick_variant(r, low);
+    const Variant v = partial_line_variant(r, pick_variant(r, low), low);
     auto* psi = static_cast<cv<T>*>(r->psi);
 #define QCS_LSW(PP, NTT, XX)                                                                                   \
     do {                                                                                                       \
