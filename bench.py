@@ -140,9 +140,12 @@ def gpu_circuit(n, layers, widths=(0, 4, 5)):
     warm-up on a smaller circuit of the same kind."""
     from qcsim_amd import QubitRegister
 
+    from qcsim_amd.tuning import apply_cache
+
     seq = random_circuit_c1(n, layers)
     total_bytes = sum(gate_bytes(g, n) for g, _ in seq)
     reg = QubitRegister(n)
+    apply_cache("c128", n)  # the sweep's launch variants, when the cache has this size
     out = {"qubits": n, "layers": layers, "gates": len(seq)}
     states = {}
     for w in widths:
@@ -199,9 +202,13 @@ def gpu_precision_c2(n, layers, width=5):
     FP64 GPU copy running it unfused; error of the former against the latter."""
     from qcsim_amd import QubitRegister
 
+    from qcsim_amd.tuning import apply_cache
+
     seq = random_circuit_c2(n, layers)
     r64 = QubitRegister(n, precision="c128")
     r32 = QubitRegister(n, precision="c64")
+    apply_cache("c64", n)  # launch variants, when the cache has this size
+    apply_cache("c128", n)
     r32.timer_begin()
     r32.apply_circuit(seq, fuse_width=width)
     ms32 = r32.timer_end()
