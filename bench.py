@@ -94,6 +94,40 @@ def gpu_qft(n, reps=3):
     }
 
 
+DENSE_CASES = {4: ([0, 1, 2, 3], [26, 27, 28, 29], [3, 8, 14, 27], [0, 1, 24, 28]),
+               5: ([0, 1, 2, 3, 4], [25, 26, 27, 28, 29], [0, 6, 9, 25, 29], [10, 11, 12, 13, 14])}
+
+
+def gpu_dense_blocks(precision="c128", n=30, reps=3, seed=1):
+    """Haar-random dense blocks of 4 and 5 qubits (the MFMA kernel) on low,
+    high and mixed qubit sets: ms per launch, HBM GB/s (2 s 2^n bytes) and
+    complex-product TFLOP/s (8 2^k FLOP per amplitude)."""
+    from qcsim_amd import QubitRegister
+    from qcsim_amd.gates import haar_unitary
+
+    rng = np.random.default_rng(seed)
+    amp = 16 if precision == "c128" else 8
+    reg = QubitRegister(n, precision=precision)
+    out = {"precision": precision, "qubits": n, "cases": []}
+    for k, sets in DENSE_CASES.items():
+        m = haar_unitary(rng, 1 << k)
+        for qs in sets:
+            qs = [q for q in qs if q < n]
+            reg.apply_dense(qs, m)
+            reg.synchronize()
+            reg.timer_begin()
+            for _ in range(reps):
+                reg.apply_dense(qs, m)
+            ms = reg.timer_end() / reps
+            out["cases"].append({
+                "k": k, "qubits": qs, "ms": round(ms, 3),
+                "GBps": round(2 * amp * (1 << n) / (ms / 1e3) / 1e9, 1),
+                "TFLOPs": round(8 * (1 << k) * (1 << n) / (ms / 1e3) / 1e12, 2),
+            })
+    reg.close()
+    return out
+
+
 def random_circuit_c1(n, layers, seed=11):
     """Config 1 circuit: per layer a Haar U(2) on every qubit, then a random
     perfect pairing with CNOT, CZ or controlled-Haar-U(2) (random orientation)."""
@@ -378,7 +412,7 @@ def main():
     ap.add_argument("--c2-layers", type=int, default=80)
     ap.add_argument("--c3", type=int, default=1024, help="config 3 batched QAOA instances (0 = skip)")
     ap.add_argument("--c4", type=int, default=1, help="config 4 learned-proposal experiment (0 = skip)")
-    ap.add_argument("--only", choices=["qft", "circuit", "c2", "c3", "c4"],
+    ap.add_argument("--only", choices=["qft", "circuit", "dense", "c2", "c3", "c4"],
                     help="run one leg alone and print its JSON (for profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -393,6 +427,7 @@ def main():
         legs = {
             "qft": lambda: gpu_qft(n),
             "circuit": lambda: gpu_circuit(n, args.circuit_layers),
+            "dense": lambda: [gpu_dense_blocks("c128", n), gpu_dense_blocks("c64", n + 2)],
             "c2": lambda: gpu_precision_c2(args.c2 or 32, args.c2_layers),
             "c3": lambda: gpu_qaoa_c3(args.c3),
             "c4": gpu_learned_c4,

@@ -400,11 +400,16 @@ class CCZGate(ThreeQubitsControlledGate):
         self._m[7, 7] = -1
 
 
-def haar_u2(rng: np.random.Generator) -> np.ndarray:
-    """Haar-random U(2): QR of a complex Ginibre matrix with the phase fix of
+def haar_unitary(rng: np.random.Generator, dim: int) -> np.ndarray:
+    """Haar-random U(dim): QR of a complex Ginibre matrix with the phase fix of
     Mezzadri, "How to generate random matrices from the classical compact
     groups", Notices AMS 54 (2007) 592-604."""
-    z = (rng.standard_normal((2, 2)) + 1j * rng.standard_normal((2, 2))) / math.sqrt(2.0)
+    z = (rng.standard_normal((dim, dim)) + 1j * rng.standard_normal((dim, dim))) / math.sqrt(2.0)
     q, r = np.linalg.qr(z)
     d = np.diagonal(r)
     return q * (d / np.abs(d))
+
+
+def haar_u2(rng: np.random.Generator) -> np.ndarray:
+    """Haar-random U(2) (haar_unitary with dim 2, same draws)."""
+    return haar_unitary(rng, 2)
