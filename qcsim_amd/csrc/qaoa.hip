@@ -219,6 +219,25 @@ __device__ void rx_round(float2* t, uint32_t tb, uint32_t lo, float c, float s) 
     __syncthreads();
 }
 
+// Per-tile terms of the cost C(s) = sum J z z + sum h z / 2 for a tile of
+// 2^tb amplitudes whose bits tb.. are fixed (by `base`): K = the part of C
+// on the fixed spins alone; F[i] (i < tb) = h_i / 2 + sum of J_ij z_j over
+// the fixed neighbours j of i; q[m] = exp(-i gamma Q(m)) with Q(m) = sum of
+// J z z over the edges inside bits 8..tb-1 when those bits hold m.
+struct TileCostLds {
+    double F[12];
+    double K;
+    float2 q[16];
+};
+
+// exp(-i th) for th in double, reduced to [-pi, pi] in double, then FP32.
+__device__ __forceinline__ float2 expmi(double th) {
+    th -= 6.283185307179586476925 * rint(th * 0.15915494309189533576888);
+    float sn, cs;
+    sincosf(float(th), &sn, &cs);
+    return make_float2(cs, -sn);
+}
+
 // One layer (cost diagonal, then Rx(2 beta) on qubits 0..tb-1) on a tile of
 // 2^tb amplitudes, 8 <= tb = 8 + M <= 12, whose element 0 has global index
 // `base`. Thread t takes elements t + 256 m (m < 2^M: the tile bits
@@ -226,24 +245,84 @@ __device__ void rx_round(float2* t, uint32_t tb, uint32_t lo, float c, float s) 
 // the (swizzled) LDS tile; then three rounds apply Rx on qubits 8..tb-1,
 // 4..7 and 0..3 with orbits held in registers. The input is |+> (amplitude
 // amp), `in` (global) or the LDS tile; the result is left in the LDS tile.
+//
+// The cost phase splits along the tile: with z_i = 1 - 2 b_i,
+//   C = K + A(t) + sum_{i in 8..tb-1} z_i F'_i(t) + Q(m),
+// A(t) = sum_{i<8} z_i F[i] + sum of J z z over edges inside bits 0..7, and
+// F'_i(t) = F[i] + sum of J_ij z_j over neighbours j < 8. So
+//   exp(-i gamma C) = exp(-i gamma (K + A + sum F')) prod_{b_i = 1}
+//                     exp(2 i gamma F'_i) q[m],
+// one sincos per factor per thread (1 + M of them for 2^M amplitudes) and,
+// per amplitude, one FP64 complex product along the Gray walk and two FP32
+// ones, instead of a sincos and an energy update per amplitude.
 template <int M>
 __device__ void tile_layer(float2* t, uint64_t base, double gamma, float c, float s, const QInst& I,
-                           const IsingLds& P, int in_mode, const float2* __restrict__ in, float amp) {
+                           const IsingLds& P, TileCostLds& TC, int in_mode, const float2* __restrict__ in, float amp) {
     constexpr uint32_t tb = 8 + M;
     const uint32_t tid = threadIdx.x;
-    uint64_t sidx = base + tid;
-    double hz, jzz;
-    ising_full(sidx, I.n, I.ne, P, hz, jzz);
+    auto zf = [](uint64_t v, uint32_t b) { return ((v >> b) & 1) ? -1.0 : 1.0; };
+    // per-tile terms
+    if (tid < tb) {
+        double f = 0.5 * P.h[tid];
+        for (uint32_t a = P.aoff[tid]; a < P.aoff[tid + 1]; ++a)
+            if (P.aj[a] >= tb) f += P.aJ[a] * zf(base, P.aj[a]);
+        TC.F[tid] = f;
+    } else if (tid == 32) {
+        double k = 0;
+        for (uint32_t i = tb; i < I.n; ++i) k += 0.5 * P.h[i] * zf(base, i);
+        for (uint32_t e = 0; e < I.ne; ++e)
+            if (P.ei[e] >= tb && P.ej[e] >= tb) k += P.J[e] * zf(base, P.ei[e]) * zf(base, P.ej[e]);
+        TC.K = k;
+    } else if (tid >= 64 && tid < 64 + (1u << M)) {
+        const uint64_t mv = uint64_t(tid - 64) << 8;
+        double qv = 0;
+        for (uint32_t e = 0; e < I.ne; ++e)
+            if (P.ei[e] >= 8 && P.ei[e] < tb && P.ej[e] >= 8 && P.ej[e] < tb)
+                qv += P.J[e] * zf(mv, P.ei[e]) * zf(mv, P.ej[e]);
+        TC.q[tid - 64] = expmi(gamma * qv);
+    }
+    __syncthreads();
+    // per-thread terms
+    double A = TC.K;
+    for (uint32_t i = 0; i < 8; ++i) {
+        const double zi = zf(tid, i);
+        A += zi * TC.F[i];
+        for (uint32_t a = P.aoff[i]; a < P.aoff[i + 1]; ++a)
+            if (P.aj[a] < i) A += P.aJ[a] * zi * zf(tid, P.aj[a]);
+    }
+    double2 e[M > 0 ? M : 1];
+    double fsum = 0;
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        const uint32_t i = 8 + k;
+        double f = TC.F[i];
+        for (uint32_t a = P.aoff[i]; a < P.aoff[i + 1]; ++a)
+            if (P.aj[a] < 8) f += P.aJ[a] * zf(tid, P.aj[a]);
+        fsum += f;
+        // exp(+2 i gamma F'_i), rescaled to modulus 1 in FP64 so that the
+        // walk's alternating products by it and its conjugate cancel
+        const float2 ek = expmi(-2.0 * gamma * f);
+        const double inv = 1.0 / sqrt(double(ek.x) * ek.x + double(ek.y) * ek.y);
+        e[k] = make_double2(ek.x * inv, ek.y * inv);
+    }
+    const float2 p0 = expmi(gamma * (A + fsum));
+    double2 ph = make_double2(p0.x, p0.y);
+#pragma unroll
     for (uint32_t m = 0; m < (1u << M); ++m) {
+        const uint32_t gm = m ^ (m >> 1);
         if (m) {
-            const uint32_t q = 8 + __builtin_ctz(m);
-            ising_flip(sidx, q, P, hz, jzz);
-            sidx ^= 1ull << q;
+            const int k = __builtin_ctz(m);
+            const double2 f = e[k];
+            const double fy = ((gm >> k) & 1) ? f.y : -f.y;  // set: times e_k; cleared: divided by it
+            ph = make_double2(ph.x * f.x - ph.y * fy, ph.x * fy + ph.y * f.x);
         }
-        const uint32_t i = uint32_t(sidx - base);
+        const float2 qm = TC.q[gm];
+        const float pr = float(ph.x), pi = float(ph.y);
+        const float wr = pr * qm.x - pi * qm.y, wi = pr * qm.y + pi * qm.x;
+        const uint32_t i = tid + (gm << 8);
         const float2 v = in_mode == IN_GLOBAL ? ldamp<float, true>(in + i)
                                               : (in_mode == IN_LDS ? t[swz(i)] : make_float2(amp, 0.f));
-        t[swz(i)] = cost_phase(v, gamma, cost_of(hz, jzz));
+        t[swz(i)] = make_float2(wr * v.x - wi * v.y, wr * v.y + wi * v.x);
     }
     __syncthreads();
     if (M > 0) rx_round<(M > 0 ? M : 1)>(t, tb, 8, c, s);
@@ -253,13 +332,13 @@ __device__ void tile_layer(float2* t, uint64_t base, double gamma, float c, floa
 
 // tile_layer for a runtime tb in 8..12.
 __device__ void tile_layer_tb(uint32_t tb, float2* t, uint64_t base, double gamma, float c, float s, const QInst& I,
-                              const IsingLds& P, int in_mode, const float2* in, float amp) {
+                              const IsingLds& P, TileCostLds& TC, int in_mode, const float2* in, float amp) {
     switch (tb) {
-        case 8: tile_layer<0>(t, base, gamma, c, s, I, P, in_mode, in, amp); break;
-        case 9: tile_layer<1>(t, base, gamma, c, s, I, P, in_mode, in, amp); break;
-        case 10: tile_layer<2>(t, base, gamma, c, s, I, P, in_mode, in, amp); break;
-        case 11: tile_layer<3>(t, base, gamma, c, s, I, P, in_mode, in, amp); break;
-        default: tile_layer<4>(t, base, gamma, c, s, I, P, in_mode, in, amp); break;
+        case 8: tile_layer<0>(t, base, gamma, c, s, I, P, TC, in_mode, in, amp); break;
+        case 9: tile_layer<1>(t, base, gamma, c, s, I, P, TC, in_mode, in, amp); break;
+        case 10: tile_layer<2>(t, base, gamma, c, s, I, P, TC, in_mode, in, amp); break;
+        case 11: tile_layer<3>(t, base, gamma, c, s, I, P, TC, in_mode, in, amp); break;
+        default: tile_layer<4>(t, base, gamma, c, s, I, P, TC, in_mode, in, amp); break;
     }
 }
 
@@ -273,6 +352,7 @@ __global__ __launch_bounds__(256) void k_qaoa_tile(float2* __restrict__ arena, c
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     float2* tile = reinterpret_cast<float2*>(smem_raw);
     __shared__ IsingLds P;
+    __shared__ TileCostLds TC;
     __shared__ double red[4];
     __shared__ uint32_t su;
     for (;;) {
@@ -293,7 +373,7 @@ __global__ __launch_bounds__(256) void k_qaoa_tile(float2* __restrict__ arena, c
                 for (uint32_t l = 0; l < p; ++l) {
                     double sb, cb;
                     sincos(an[2 * l + 1], &sb, &cb);
-                    tile_layer_tb(I.n, tile, 0, an[2 * l], float(cb), float(sb), I, P, l ? IN_LDS : IN_PLUS,
+                    tile_layer_tb(I.n, tile, 0, an[2 * l], float(cb), float(sb), I, P, TC, l ? IN_LDS : IN_PLUS,
                                   nullptr, amp);
                 }
             } else {
@@ -331,7 +411,7 @@ __global__ __launch_bounds__(256) void k_qaoa_tile(float2* __restrict__ arena, c
             double sb, cb;
             sincos(an[2 * U.layer + 1], &sb, &cb);
             if (tb >= 8) {
-                tile_layer_tb(tb, tile, base, an[2 * U.layer], float(cb), float(sb), I, P,
+                tile_layer_tb(tb, tile, base, an[2 * U.layer], float(cb), float(sb), I, P, TC,
                               U.layer ? IN_GLOBAL : IN_PLUS, psi + base, amp);
             } else {
                 for (uint32_t i = threadIdx.x; i < C; i += 256)
