@@ -435,6 +435,7 @@ __global__ __launch_bounds__(256) void k_qaoa_mix(float2* __restrict__ arena, co
                                                   uint32_t p, IsingArgsDev ga, double* __restrict__ partial) {
     constexpr int D = 1 << g;
     __shared__ IsingLds P;
+    __shared__ double qe[D];  // E: -sum J z z over the edges inside the group, per group value
     __shared__ double red[4];
     __shared__ uint32_t su;
     for (;;) {
@@ -445,7 +446,19 @@ __global__ __launch_bounds__(256) void k_qaoa_mix(float2* __restrict__ arena, co
         if (u >= nunits) break;
         const QUnit U = units[u];
         const QInst I = insts[U.inst];
-        if (E) load_params(P, I, ga);
+        if (E) {
+            load_params(P, I, ga);
+            for (uint32_t m = threadIdx.x; m < uint32_t(D); m += 256) {
+                double v = 0;
+                for (uint32_t e = 0; e < I.ne; ++e) {
+                    const uint32_t a = P.ei[e], b = P.ej[e];
+                    if (a >= U.qlo && a < U.qlo + g && b >= U.qlo && b < U.qlo + g)
+                        v -= (((m >> (a - U.qlo)) ^ (m >> (b - U.qlo))) & 1) ? -P.J[e] : P.J[e];
+                }
+                qe[m] = v;
+            }
+            __syncthreads();
+        }
         float2* psi = arena + I.off;
         double sb, cb;
         sincos(ang[(size_t(U.inst) * p + U.layer) * 2 + 1], &sb, &cb);
@@ -466,22 +479,44 @@ __global__ __launch_bounds__(256) void k_qaoa_mix(float2* __restrict__ arena, co
                 for (int c = 0; c < D; ++c)
                     if (!((c >> q) & 1)) rx_pair(x[c], x[c | (1 << q)], fc, fs);
             if (E) {
-                uint64_t s = base;
+                // E over the orbit: E(base) in full, then along a Gray walk
+                // over the group bits: setting bit k of the group (spin
+                // i = qlo + k) adds 2 (h_i + sum of J_ij z_j over neighbours
+                // j outside the group), clearing it subtracts that; the
+                // edges inside the group come from the table qe.
                 double hz, jzz;
-                ising_full(s, I.n, I.ne, P, hz, jzz);
+                ising_full(base, I.n, I.ne, P, hz, jzz);
+                const double e0 = energy_of(hz, jzz) - qe[0];
+                double g2[g];
+#pragma unroll
+                for (int k = 0; k < g; ++k) {
+                    const uint32_t i = U.qlo + k;
+                    double f = P.h[i];
+                    for (uint32_t a = P.aoff[i]; a < P.aoff[i + 1]; ++a) {
+                        const uint32_t j = P.aj[a];
+                        if (j < U.qlo || j >= U.qlo + g) f += ((base >> j) & 1) ? -P.aJ[a] : P.aJ[a];
+                    }
+                    g2[k] = 2.0 * f;
+                }
+                double dsum = 0;
 #pragma unroll
                 for (int m = 0; m < D; ++m) {
-                    if (m) {
-                        const uint32_t q = U.qlo + __builtin_ctz(uint32_t(m));
-                        ising_flip(s, q, P, hz, jzz);
-                        s ^= 1ull << q;
-                    }
                     const int c = m ^ (m >> 1);
-                    acc += (double(x[c].x) * x[c].x + double(x[c].y) * x[c].y) * energy_of(hz, jzz);
+                    if (m) {
+                        const int k = __builtin_ctz(uint32_t(m));
+                        dsum += ((c >> k) & 1) ? g2[k] : -g2[k];
+                    }
+                    acc += (double(x[c].x) * x[c].x + double(x[c].y) * x[c].y) * (e0 + dsum + qe[c]);
                 }
             }
+            // The store addresses are recomputed from a copy of `base` the
+            // compiler cannot see through, so that it does not keep all 2^g
+            // 64-bit load addresses live across the rotations (for g = 6 that
+            // alone would be 128 VGPRs).
+            uint64_t sbase = base;
+            asm volatile("" : "+v"(sbase));
 #pragma unroll
-            for (int c = 0; c < D; ++c) stamp<float, true>(psi + (base | (uint64_t(c) << U.qlo)), x[c]);
+            for (int c = 0; c < D; ++c) stamp<float, true>(psi + (sbase | (uint64_t(c) << U.qlo)), x[c]);
         }
         if (E) {
             const double tot = block_sum(acc, red);
