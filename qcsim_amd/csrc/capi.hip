@@ -136,7 +136,8 @@ int qcs_create(uint32_t nqubits, int precision, qcs_register** out) {
     }
     r->mat_bytes = sizeof(double) * 64 * 64;  // real embedding of a 32x32 complex block
     if (hipMalloc(reinterpret_cast<void**>(&r->red), sizeof(double) * (kRedBlocks * 3 + 64)) != hipSuccess ||
-        hipMalloc(reinterpret_cast<void**>(&r->chunk), sizeof(double) * nchunks) != hipSuccess ||
+        hipMalloc(reinterpret_cast<void**>(&r->chunk), sizeof(double) * (nchunks + (nchunks + 2047) / 2048)) !=
+            hipSuccess ||
         hipMalloc(reinterpret_cast<void**>(&r->dout), sizeof(uint64_t) * 64) != hipSuccess ||
         hipMalloc(&r->mat, r->mat_bytes * qcs_register::kMatSlots) != hipSuccess) {
         hipGetLastError();

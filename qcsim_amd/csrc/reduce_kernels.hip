@@ -123,14 +123,26 @@ struct ChunkGeom {
 // [t*per, (t+1)*per); returns the exclusive prefix of thread t within the
 // chunk and the chunk total (to all threads). Identical code runs in the
 // chunk pass and in the search pass, so both see bit-identical values.
+// The norms are first staged in LDS by coalesced loads (consecutive lanes on
+// consecutive amplitudes; a thread's own run of `per` amplitudes would put
+// its lanes `per` amplitudes apart), one pad slot per 8 so that the
+// per-thread runs fall in different banks; each thread then adds its run
+// in index order, as before.
+constexpr uint32_t kChunkMax = 1u << kChunkLog;
+constexpr uint32_t kNrmLds = kChunkMax + kChunkMax / 8;
+
+__device__ __forceinline__ uint32_t nrm_slot(uint64_t e) { return uint32_t(e + (e >> 3)); }
+
 template <typename T>
 __device__ __forceinline__ double chunk_thread_prefix(const cv<T>* __restrict__ psi, uint64_t cbase,
-                                                      ChunkGeom g, double* lds, double* total) {
+                                                      ChunkGeom g, double* lds, double* nrm, double* total) {
+    for (uint64_t i = threadIdx.x; i < g.clen; i += kScanThreads) nrm[nrm_slot(i)] = norm2<T>(psi[cbase + i]);
+    __syncthreads();
     double mine = 0;
     const uint64_t beg = uint64_t(threadIdx.x) * g.per;
     for (uint32_t i = 0; i < g.per; ++i) {
         const uint64_t e = beg + i;
-        if (e < g.clen) mine += norm2<T>(psi[cbase + e]);
+        if (e < g.clen) mine += nrm[nrm_slot(e)];
     }
     // inclusive wave scan
     double v = mine;
@@ -156,13 +168,61 @@ template <typename T>
 __global__ __launch_bounds__(kScanThreads) void k_chunk_sums(const cv<T>* __restrict__ psi, ChunkGeom g,
                                                              double* __restrict__ chunk, uint64_t c0) {
     __shared__ double lds[4];
+    __shared__ double nrm[kNrmLds];
     double total;
     const uint64_t c = c0 + blockIdx.x;
-    chunk_thread_prefix<T>(psi, c * g.clen, g, lds, &total);
+    chunk_thread_prefix<T>(psi, c * g.clen, g, lds, nrm, &total);
     if (threadIdx.x == 0) chunk[c] = total;
 }
 
-// In-place inclusive scan of chunk totals by one block of 1024 threads.
+// Chunk totals are scanned in groups of kScanGroup by many blocks (each
+// thread adds a run of 8 in index order, then the block scans the thread
+// sums), the group totals by one block, and the group offsets are added
+// back: the same fixed order on every run.
+constexpr uint32_t kScanGroup = 2048;
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_group(double* __restrict__ chunk, uint64_t nchunks,
+                                                             double* __restrict__ grp) {
+    __shared__ double lds[4];
+    constexpr uint32_t per = kScanGroup / kScanThreads;
+    const uint64_t beg = uint64_t(blockIdx.x) * kScanGroup + uint64_t(threadIdx.x) * per;
+    double v[per];
+    double run = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < per; ++i) {
+        const uint64_t e = beg + i;
+        run += e < nchunks ? chunk[e] : 0.0;
+        v[i] = run;
+    }
+    double x = run;
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const double n = __shfl_up(x, o, 64);
+        if (lane >= o) x += n;
+    }
+    double off = __shfl_up(x, 1, 64);
+    if (lane == 0) off = 0;
+    const int w = threadIdx.x >> 6;
+    if (lane == 63) lds[w] = x;
+    __syncthreads();
+    for (int k = 0; k < w; ++k) off += lds[k];
+#pragma unroll
+    for (uint32_t i = 0; i < per; ++i) {
+        const uint64_t e = beg + i;
+        if (e < nchunks) chunk[e] = off + v[i];
+    }
+    if (threadIdx.x == kScanThreads - 1) grp[blockIdx.x] = off + run;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_add_group_offsets(double* __restrict__ chunk, uint64_t nchunks,
+                                                                    const double* __restrict__ grp) {
+    const uint64_t e = uint64_t(blockIdx.x) * kScanThreads + threadIdx.x + kScanGroup;  // group 0 adds 0
+    if (e < nchunks) chunk[e] += grp[e / kScanGroup - 1];
+}
+
+// In-place inclusive scan of up to a few thousand values (the group totals)
+// by one block of 1024 threads.
 __global__ __launch_bounds__(1024) void k_scan_chunks(double* __restrict__ chunk, uint64_t nchunks) {
     __shared__ double lds[16];
     const uint64_t per = (nchunks + 1023) / 1024;
@@ -202,6 +262,7 @@ __global__ __launch_bounds__(kScanThreads) void k_search(const cv<T>* __restrict
                                                          const double* __restrict__ probs, int strict,
                                                          uint64_t dflt, uint64_t* __restrict__ out) {
     __shared__ double lds[4];
+    __shared__ double nrm[kNrmLds];
     __shared__ uint64_t sc;
     __shared__ unsigned long long found;
     const double prob = probs[blockIdx.x];
@@ -226,14 +287,14 @@ __global__ __launch_bounds__(kScanThreads) void k_search(const cv<T>* __restrict
     }
     double total;
     const uint64_t cbase = c * g.clen;
-    const double pre = chunk_thread_prefix<T>(psi, cbase, g, lds, &total);
+    const double pre = chunk_thread_prefix<T>(psi, cbase, g, lds, nrm, &total);
     double acc = (c > 0 ? chunk[c - 1] : 0.0) + pre;
     const uint64_t beg = uint64_t(threadIdx.x) * g.per;
     double run = 0;
     for (uint32_t i = 0; i < g.per; ++i) {
         const uint64_t e = beg + i;
         if (e >= g.clen) break;
-        run += norm2<T>(psi[cbase + e]);
+        run += nrm[nrm_slot(e)];
         const double v = acc + run;
         const bool ok = strict ? (v > prob) : (v >= prob);
         if (ok) {
@@ -403,9 +464,23 @@ static int cdf_search_t(qcs_register* r, uint64_t nshots, const double* probs_ho
         e = check_launch(r, "k_chunk_sums");
         if (e) return e;
     }
-    hipLaunchKernelGGL(k_scan_chunks, dim3(1), dim3(1024), 0, r->stream, r->chunk, nchunks);
-    e = check_launch(r, "k_scan_chunks");
+    // group totals live after the chunk totals (the buffer holds both)
+    const uint64_t ngroups = (nchunks + kScanGroup - 1) / kScanGroup;
+    double* grp = r->chunk + nchunks;
+    hipLaunchKernelGGL(k_scan_group, dim3(unsigned(ngroups)), dim3(kScanThreads), 0, r->stream, r->chunk, nchunks,
+                       grp);
+    e = check_launch(r, "k_scan_group");
     if (e) return e;
+    if (ngroups > 1) {
+        hipLaunchKernelGGL(k_scan_chunks, dim3(1), dim3(1024), 0, r->stream, grp, ngroups);
+        e = check_launch(r, "k_scan_chunks");
+        if (e) return e;
+        const uint64_t nadd = nchunks - kScanGroup;
+        hipLaunchKernelGGL(k_add_group_offsets, dim3(unsigned((nadd + kScanThreads - 1) / kScanThreads)),
+                           dim3(kScanThreads), 0, r->stream, r->chunk, nchunks, grp);
+        e = check_launch(r, "k_add_group_offsets");
+        if (e) return e;
+    }
     // per-shot buffers
     double* dprobs = nullptr;
     uint64_t* douts = nullptr;
