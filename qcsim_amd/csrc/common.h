@@ -151,7 +151,7 @@ struct qcs_register {
 
 namespace qcs {
 
-constexpr int kRedBlocks = 1024;     // fixed grid of reductions (deterministic)
+constexpr int kRedBlocks = 4096;     // fixed grid of reductions (deterministic)
 constexpr int kRedThreads = 256;
 constexpr uint32_t kChunkLog = 11;   // CDF chunk = 2048 amplitudes
 constexpr int kScanThreads = 256;

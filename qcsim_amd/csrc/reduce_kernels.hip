@@ -58,8 +58,9 @@ __global__ __launch_bounds__(kRedThreads) void k_sum_sq(const cv<T>* __restrict_
     __shared__ double lds[4];
     double acc = 0;
     const uint64_t stride = uint64_t(gridDim.x) * kRedThreads;
+#pragma unroll 4
     for (uint64_t o = uint64_t(blockIdx.x) * kRedThreads + threadIdx.x; o < norb; o += stride)
-        acc += norm2<T>(psi[insert_run(o, first, len) | val]);
+        acc += norm2<T>(ldamp<T, true>(psi + (insert_run(o, first, len) | val)));
     const double t = block_sum256(acc, lds);
     if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
@@ -82,9 +83,10 @@ __global__ __launch_bounds__(kRedThreads) void k_inner(const cv<Ta>* __restrict_
     __shared__ double lds[4];
     double re = 0, im = 0, mx = 0;
     const uint64_t stride = uint64_t(gridDim.x) * kRedThreads;
+#pragma unroll 4
     for (uint64_t s = uint64_t(blockIdx.x) * kRedThreads + threadIdx.x; s < A; s += stride) {
-        const cv<Ta> x = a[s];
-        const cv<Tb> y = b[s];
+        const cv<Ta> x = ldamp<Ta, true>(a + s);
+        const cv<Tb> y = ldamp<Tb, true>(b + s);
         const double xr = x.x, xi = x.y, yr = y.x, yi = y.y;
         re += xr * yr + xi * yi;  // conj(x) * y
         im += xr * yi - xi * yr;
@@ -104,11 +106,11 @@ __global__ __launch_bounds__(kRedThreads) void k_inner(const cv<Ta>* __restrict_
     }
 }
 
-__global__ void k_final_max(const double* __restrict__ partial, int n, double* __restrict__ out) {
-    if (threadIdx.x != 0) return;
+__global__ __launch_bounds__(64) void k_final_max(const double* __restrict__ partial, int n, double* __restrict__ out) {
     double m = 0;
-    for (int i = 0; i < n; ++i) m = fmax(m, partial[i * 3 + 2]);
-    out[2] = m;
+    for (int i = threadIdx.x; i < n; i += 64) m = fmax(m, partial[i * 3 + 2]);
+    m = wave_max(m);
+    if (threadIdx.x == 0) out[2] = m;
 }
 
 // ---------------------------------------------------------------------------
