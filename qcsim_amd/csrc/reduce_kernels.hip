@@ -313,20 +313,20 @@ __global__ __launch_bounds__(kScanThreads) void k_search(const cv<T>* __restrict
 
 template <typename T>
 __global__ __launch_bounds__(kRedThreads) void k_collapse(cv<T>* __restrict__ psi, uint64_t A, uint64_t mask,
-                                                          uint64_t val, T scale) {
-    const uint64_t stride = uint64_t(gridDim.x) * kRedThreads;
-    for (uint64_t s = uint64_t(blockIdx.x) * kRedThreads + threadIdx.x; s < A; s += stride) {
-        cv<T> v;
-        if ((s & mask) == val) {
-            v = psi[s];
-            v.x *= scale;
-            v.y *= scale;
-        } else {
-            v.x = 0;
-            v.y = 0;
-        }
-        psi[s] = v;
+                                                          uint64_t val, T scale, uint64_t boff) {
+    // one amplitude per lane, one pass (no grid-stride loop)
+    const uint64_t s = (uint64_t(blockIdx.x) + boff) * kRedThreads + threadIdx.x;
+    if (s >= A) return;
+    cv<T> v;
+    if ((s & mask) == val) {
+        v = ldamp<T, true>(psi + s);
+        v.x *= scale;
+        v.y *= scale;
+    } else {
+        v.x = 0;
+        v.y = 0;
     }
+    stamp<T, true>(psi + s, v);
 }
 
 // Exact Ising energy sum_s |psi_s|^2 E(s), E(s) = -(sum h z + sum J z z).
@@ -512,14 +512,20 @@ int cdf_search(qcs_register* r, uint64_t nshots, const double* probs_host, int s
 }
 
 int kcollapse(qcs_register* r, uint64_t mask, uint64_t val, double scale) {
-    const int nb = 4096;
-    if (r->prec == QCS_COMPLEX128)
-        hipLaunchKernelGGL((k_collapse<double>), dim3(nb), dim3(kRedThreads), 0, r->stream,
-                           static_cast<double2*>(r->psi), r->A, mask, val, scale);
-    else
-        hipLaunchKernelGGL((k_collapse<float>), dim3(nb), dim3(kRedThreads), 0, r->stream,
-                           static_cast<float2*>(r->psi), r->A, mask, val, float(scale));
-    return check_launch(r, "k_collapse");
+    // at most 2^22 blocks (2^30 work-items) per launch: a grid's size in
+    // work-items must stay below 2^32
+    const uint64_t nblk = (r->A + kRedThreads - 1) / kRedThreads;
+    for (uint64_t boff = 0; boff < nblk; boff += (1ull << 22)) {
+        const unsigned g = unsigned(std::min<uint64_t>(1ull << 22, nblk - boff));
+        if (r->prec == QCS_COMPLEX128)
+            hipLaunchKernelGGL((k_collapse<double>), dim3(g), dim3(kRedThreads), 0, r->stream,
+                               static_cast<double2*>(r->psi), r->A, mask, val, scale, boff);
+        else
+            hipLaunchKernelGGL((k_collapse<float>), dim3(g), dim3(kRedThreads), 0, r->stream,
+                               static_cast<float2*>(r->psi), r->A, mask, val, float(scale), boff);
+        if (const int e = check_launch(r, "k_collapse")) return e;
+    }
+    return QCS_OK;
 }
 
 template <typename T>
