@@ -262,20 +262,24 @@ __global__ __launch_bounds__(kThreads) void k_3q(cv<T>* __restrict__ psi, uint64
     }
 }
 
+// Whole-state sweeps (fill, scale, full diagonal): one amplitude per lane
+// in one pass, nontemporal, in launches of at most kMaxBlocks blocks that
+// start at block `boff`.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_fill(cv<T>* __restrict__ psi, uint64_t A, T re, T im) {
-    const uint64_t stride = uint64_t(gridDim.x) * kThreads;
+__global__ __launch_bounds__(kThreads) void k_fill(cv<T>* __restrict__ psi, uint64_t A, T re, T im, uint64_t boff) {
+    const uint64_t i = (uint64_t(blockIdx.x) + boff) * kThreads + threadIdx.x;
+    if (i >= A) return;
     cv<T> v;
     v.x = re;
     v.y = im;
-    for (uint64_t i = uint64_t(blockIdx.x) * kThreads + threadIdx.x; i < A; i += stride) psi[i] = v;
+    stamp<T, true>(psi + i, v);
 }
 
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_scale(cv<T>* __restrict__ psi, uint64_t A, T re, T im) {
-    const uint64_t stride = uint64_t(gridDim.x) * kThreads;
-    for (uint64_t i = uint64_t(blockIdx.x) * kThreads + threadIdx.x; i < A; i += stride)
-        psi[i] = cmul<T>(re, im, psi[i]);
+__global__ __launch_bounds__(kThreads) void k_scale(cv<T>* __restrict__ psi, uint64_t A, T re, T im, uint64_t boff) {
+    const uint64_t i = (uint64_t(blockIdx.x) + boff) * kThreads + threadIdx.x;
+    if (i >= A) return;
+    stamp<T, true>(psi + i, cmul<T>(re, im, ldamp<T, true>(psi + i)));
 }
 
 template <typename T>
@@ -288,24 +292,15 @@ __global__ void k_set_one(cv<T>* __restrict__ psi, uint64_t idx, T re, T im) {
 
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_diag_full(cv<T>* __restrict__ psi, uint64_t A,
-                                                        const double2* __restrict__ d) {
-    const uint64_t stride = uint64_t(gridDim.x) * kThreads;
-    for (uint64_t i = uint64_t(blockIdx.x) * kThreads + threadIdx.x; i < A; i += stride) {
-        const double2 f = d[i];
-        psi[i] = cmul<T>(T(f.x), T(f.y), psi[i]);
-    }
+                                                        const double2* __restrict__ d, uint64_t boff) {
+    const uint64_t i = (uint64_t(blockIdx.x) + boff) * kThreads + threadIdx.x;
+    if (i >= A) return;
+    const double2 f = ldamp<double, true>(d + i);
+    stamp<T, true>(psi + i, cmul<T>(T(f.x), T(f.y), ldamp<T, true>(psi + i)));
 }
 
 // ---------------------------------------------------------------------------
 // Host launchers.
-
-static unsigned stream_grid(uint64_t A) {
-    const uint64_t per = uint64_t(kThreads) * 8;
-    uint64_t g = (A + per - 1) / per;
-    if (g > 65536) g = 65536;
-    if (g < 1) g = 1;
-    return unsigned(g);
-}
 
 // Gates that touch only the amplitudes with some bit set (controls, phase
 // bits, the a != b half of a swap) use only part of each 128-byte line when
@@ -546,24 +541,26 @@ int k3q(qcs_register* r, uint32_t t0, uint32_t t1, uint32_t t2, const hcplx* m) 
 }
 
 int kfill(qcs_register* r, double re, double im) {
-    const unsigned g = stream_grid(r->A);
-    if (r->prec == QCS_COMPLEX128)
-        hipLaunchKernelGGL((k_fill<double>), dim3(g), dim3(kThreads), 0, r->stream,
-                           static_cast<double2*>(r->psi), r->A, re, im);
-    else
-        hipLaunchKernelGGL((k_fill<float>), dim3(g), dim3(kThreads), 0, r->stream,
-                           static_cast<float2*>(r->psi), r->A, float(re), float(im));
+    QCS_CHUNKED(grid_for(r->A, kThreads), {
+        if (r->prec == QCS_COMPLEX128)
+            hipLaunchKernelGGL((k_fill<double>), dim3(g), dim3(kThreads), 0, r->stream,
+                               static_cast<double2*>(r->psi), r->A, re, im, boff);
+        else
+            hipLaunchKernelGGL((k_fill<float>), dim3(g), dim3(kThreads), 0, r->stream,
+                               static_cast<float2*>(r->psi), r->A, float(re), float(im), boff);
+    })
     return check_launch(r, "k_fill");
 }
 
 int kscale(qcs_register* r, double re, double im) {
-    const unsigned g = stream_grid(r->A);
-    if (r->prec == QCS_COMPLEX128)
-        hipLaunchKernelGGL((k_scale<double>), dim3(g), dim3(kThreads), 0, r->stream,
-                           static_cast<double2*>(r->psi), r->A, re, im);
-    else
-        hipLaunchKernelGGL((k_scale<float>), dim3(g), dim3(kThreads), 0, r->stream,
-                           static_cast<float2*>(r->psi), r->A, float(re), float(im));
+    QCS_CHUNKED(grid_for(r->A, kThreads), {
+        if (r->prec == QCS_COMPLEX128)
+            hipLaunchKernelGGL((k_scale<double>), dim3(g), dim3(kThreads), 0, r->stream,
+                               static_cast<double2*>(r->psi), r->A, re, im, boff);
+        else
+            hipLaunchKernelGGL((k_scale<float>), dim3(g), dim3(kThreads), 0, r->stream,
+                               static_cast<float2*>(r->psi), r->A, float(re), float(im), boff);
+    })
     return check_launch(r, "k_scale");
 }
 
@@ -578,13 +575,14 @@ int kset_one(qcs_register* r, uint64_t idx, double re, double im) {
 }
 
 int kdiag_full(qcs_register* r, const double* diag_dev) {
-    const unsigned g = stream_grid(r->A);
-    if (r->prec == QCS_COMPLEX128)
-        hipLaunchKernelGGL((k_diag_full<double>), dim3(g), dim3(kThreads), 0, r->stream,
-                           static_cast<double2*>(r->psi), r->A, reinterpret_cast<const double2*>(diag_dev));
-    else
-        hipLaunchKernelGGL((k_diag_full<float>), dim3(g), dim3(kThreads), 0, r->stream,
-                           static_cast<float2*>(r->psi), r->A, reinterpret_cast<const double2*>(diag_dev));
+    QCS_CHUNKED(grid_for(r->A, kThreads), {
+        if (r->prec == QCS_COMPLEX128)
+            hipLaunchKernelGGL((k_diag_full<double>), dim3(g), dim3(kThreads), 0, r->stream,
+                               static_cast<double2*>(r->psi), r->A, reinterpret_cast<const double2*>(diag_dev), boff);
+        else
+            hipLaunchKernelGGL((k_diag_full<float>), dim3(g), dim3(kThreads), 0, r->stream,
+                               static_cast<float2*>(r->psi), r->A, reinterpret_cast<const double2*>(diag_dev), boff);
+    })
     return check_launch(r, "k_diag_full");
 }
 
