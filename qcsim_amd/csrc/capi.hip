@@ -517,7 +517,7 @@ int qcs_sample(qcs_register* r, int all, uint32_t first, uint32_t second, uint64
     const uint64_t len = t >= r->A ? r->A : t + 1;
     std::vector<double> probs(nshots);
     for (uint64_t s = 0; s < nshots; ++s) probs[s] = 1. - u[s];
-    e = cdf_search(r, nshots, probs.data(), 0, r->A, outcomes, 1);
+    e = cdf_search(r, nshots, probs.data(), 0, r->A, outcomes, 0);  // the CDF just built
     if (e) return e;
     for (uint64_t s = 0; s < nshots; ++s) {
         const uint64_t v = outcomes[s] < len ? outcomes[s] : len;  // lower_bound on [0, len)

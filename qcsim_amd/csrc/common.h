@@ -199,8 +199,13 @@ int kdense(qcs_register* r, uint32_t k, const uint32_t* qubits, const double* ma
 // reduce_kernels.hip (results in FP64)
 int rsum_sq(qcs_register* r, uint32_t first, uint32_t len, uint64_t val, double* out);
 int rinner(qcs_register* a, qcs_register* b, double* re_im, double* max_abs);
+// CDF search: out[s] = first index whose cumulative probability reaches
+// u_host[s] (exceeds it with `strict`), or `dflt`. build_cdf = 0 reuses the
+// chunk CDF built by the previous search on this register, which is only
+// valid if the state has not changed since (the norms and rsum_sq also use
+// that buffer).
 int cdf_search(qcs_register* r, uint64_t nshots, const double* u_host, int strict,
-               uint64_t dflt, uint64_t* out_host, int last_in_chunk_fallback);
+               uint64_t dflt, uint64_t* out_host, int build_cdf);
 int kcollapse(qcs_register* r, uint64_t mask, uint64_t val, double scale);
 int rising(qcs_register* r, uint32_t ne, const uint32_t* ei, const uint32_t* ej,
            const double* J, const double* h, double* energy);

@@ -65,6 +65,39 @@ __global__ __launch_bounds__(kRedThreads) void k_sum_sq(const cv<T>* __restrict_
     if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
 
+// Large sums in one pass: block b adds the 2048 matching amplitudes
+// 2048 b .. 2048 b + 2047 (8 per thread, 256 apart: consecutive lanes on
+// consecutive amplitudes), then k_sum_partials adds the block sums with the
+// fixed grid and k_final_sum the grid's. Fixed order, so deterministic.
+constexpr uint32_t kSumBlock = 2048;
+
+template <typename T>
+__global__ __launch_bounds__(kRedThreads) void k_sum_sq_blocks(const cv<T>* __restrict__ psi, uint64_t norb,
+                                                               uint32_t first, uint32_t len, uint64_t val,
+                                                               double* __restrict__ partial, uint64_t b0) {
+    __shared__ double lds[4];
+    const uint64_t blk = b0 + blockIdx.x;
+    const uint64_t o0 = blk * kSumBlock + threadIdx.x;
+    double acc = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kSumBlock / kRedThreads; ++k) {
+        const uint64_t o = o0 + uint64_t(k) * kRedThreads;
+        if (o < norb) acc += norm2<T>(ldamp<T, true>(psi + (insert_run(o, first, len) | val)));
+    }
+    const double t = block_sum256(acc, lds);
+    if (threadIdx.x == 0) partial[blk] = t;
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_sum_partials(const double* __restrict__ in, uint64_t n,
+                                                              double* __restrict__ partial) {
+    __shared__ double lds[4];
+    double acc = 0;
+    const uint64_t stride = uint64_t(gridDim.x) * kRedThreads;
+    for (uint64_t i = uint64_t(blockIdx.x) * kRedThreads + threadIdx.x; i < n; i += stride) acc += in[i];
+    const double t = block_sum256(acc, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
 // Fixed-order sum of `n` partials (n <= kRedBlocks) by one block.
 __global__ __launch_bounds__(kRedThreads) void k_final_sum(const double* __restrict__ partial, int n,
                                                            int stride_elems, double* __restrict__ out) {
@@ -387,11 +420,26 @@ static int fetch(qcs_register* r, const double* dev, int n, double* host) {
 template <typename T>
 static int rsum_sq_t(qcs_register* r, uint32_t first, uint32_t len, uint64_t val, double* out) {
     const uint64_t norb = r->A >> len;
-    const int nb = red_blocks(norb);
-    hipLaunchKernelGGL((k_sum_sq<T>), dim3(nb), dim3(kRedThreads), 0, r->stream,
-                       static_cast<const cv<T>*>(r->psi), norb, first, len, val, r->red);
-    int e = check_launch(r, "k_sum_sq");
-    if (e) return e;
+    int nb = 0, e = 0;
+    if (norb >= (uint64_t(1) << 20)) {
+        // block sums go to the CDF buffer (2^n / 2048 doubles, scratch
+        // between searches), then the fixed grid adds them
+        const uint64_t nblk = (norb + kSumBlock - 1) / kSumBlock;
+        for (uint64_t b0 = 0; b0 < nblk; b0 += (1ull << 22)) {
+            const unsigned g = unsigned(std::min<uint64_t>(1ull << 22, nblk - b0));
+            hipLaunchKernelGGL((k_sum_sq_blocks<T>), dim3(g), dim3(kRedThreads), 0, r->stream,
+                               static_cast<const cv<T>*>(r->psi), norb, first, len, val, r->chunk, b0);
+            if ((e = check_launch(r, "k_sum_sq_blocks"))) return e;
+        }
+        nb = red_blocks(nblk);
+        hipLaunchKernelGGL(k_sum_partials, dim3(nb), dim3(kRedThreads), 0, r->stream, r->chunk, nblk, r->red);
+        if ((e = check_launch(r, "k_sum_partials"))) return e;
+    } else {
+        nb = red_blocks(norb);
+        hipLaunchKernelGGL((k_sum_sq<T>), dim3(nb), dim3(kRedThreads), 0, r->stream,
+                           static_cast<const cv<T>*>(r->psi), norb, first, len, val, r->red);
+        if ((e = check_launch(r, "k_sum_sq"))) return e;
+    }
     double* res = r->red + kRedBlocks * 3;
     hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(kRedThreads), 0, r->stream, r->red, nb, 1, res);
     e = check_launch(r, "k_final_sum");
@@ -452,14 +500,14 @@ struct ShotBuf {
 
 template <typename T>
 static int cdf_search_t(qcs_register* r, uint64_t nshots, const double* probs_host, int strict,
-                        uint64_t dflt, uint64_t* out_host) {
+                        uint64_t dflt, uint64_t* out_host, bool build) {
     const ChunkGeom g = chunk_geom(r);
     const uint64_t nchunks = r->A / g.clen;
     const auto* psi = static_cast<const cv<T>*>(r->psi);
     // at most 2^22 chunks (2^30 work-items) per launch: a grid's size in
     // work-items must stay below 2^32
     int e = 0;
-    for (uint64_t c0 = 0; c0 < nchunks; c0 += (1ull << 22)) {
+    for (uint64_t c0 = 0; build && c0 < nchunks; c0 += (1ull << 22)) {
         const uint64_t cnt = std::min<uint64_t>(1ull << 22, nchunks - c0);
         hipLaunchKernelGGL((k_chunk_sums<T>), dim3(unsigned(cnt)), dim3(kScanThreads), 0, r->stream, psi, g,
                            r->chunk, c0);
@@ -469,11 +517,13 @@ static int cdf_search_t(qcs_register* r, uint64_t nshots, const double* probs_ho
     // group totals live after the chunk totals (the buffer holds both)
     const uint64_t ngroups = (nchunks + kScanGroup - 1) / kScanGroup;
     double* grp = r->chunk + nchunks;
-    hipLaunchKernelGGL(k_scan_group, dim3(unsigned(ngroups)), dim3(kScanThreads), 0, r->stream, r->chunk, nchunks,
-                       grp);
-    e = check_launch(r, "k_scan_group");
-    if (e) return e;
-    if (ngroups > 1) {
+    if (build) {
+        hipLaunchKernelGGL(k_scan_group, dim3(unsigned(ngroups)), dim3(kScanThreads), 0, r->stream, r->chunk,
+                           nchunks, grp);
+        e = check_launch(r, "k_scan_group");
+        if (e) return e;
+    }
+    if (build && ngroups > 1) {
         hipLaunchKernelGGL(k_scan_chunks, dim3(1), dim3(1024), 0, r->stream, grp, ngroups);
         e = check_launch(r, "k_scan_chunks");
         if (e) return e;
@@ -505,10 +555,10 @@ static int cdf_search_t(qcs_register* r, uint64_t nshots, const double* probs_ho
 }
 
 int cdf_search(qcs_register* r, uint64_t nshots, const double* probs_host, int strict, uint64_t dflt,
-               uint64_t* out_host, int /*last_in_chunk_fallback*/) {
+               uint64_t* out_host, int build_cdf) {
     if (nshots == 0) return QCS_OK;
-    return r->prec == QCS_COMPLEX128 ? cdf_search_t<double>(r, nshots, probs_host, strict, dflt, out_host)
-                                     : cdf_search_t<float>(r, nshots, probs_host, strict, dflt, out_host);
+    return r->prec == QCS_COMPLEX128 ? cdf_search_t<double>(r, nshots, probs_host, strict, dflt, out_host, build_cdf)
+                                     : cdf_search_t<float>(r, nshots, probs_host, strict, dflt, out_host, build_cdf);
 }
 
 int kcollapse(qcs_register* r, uint64_t mask, uint64_t val, double scale) {
