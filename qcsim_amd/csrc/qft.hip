@@ -94,14 +94,30 @@ __device__ __forceinline__ cv<T> mul_ph(cv<T> v, double2 p, bool conj) {
     return r;
 }
 
+// Unnormalised Hadamard butterfly (a + b, a - b): each pass applies the
+// 2^(-1/2) of all its Hadamards once, to every amplitude it stores
+// (hscale), instead of once per butterfly.
 template <typename T>
 __device__ __forceinline__ void hadamard(cv<T>& a, cv<T>& b) {
-    const T s = T(0.70710678118654752440);
     const cv<T> x = a, y = b;
-    a.x = (x.x + y.x) * s;
-    a.y = (x.y + y.y) * s;
-    b.x = (x.x - y.x) * s;
-    b.y = (x.y - y.y) * s;
+    a.x = x.x + y.x;
+    a.y = x.y + y.y;
+    b.x = x.x - y.x;
+    b.y = x.y - y.y;
+}
+
+// 2^(-h/2): the normalisation of h Hadamards.
+template <typename T>
+__device__ __forceinline__ T hscale(uint32_t h) {
+    const T p = T(ldexp(1.0, -int(h / 2)));
+    return (h & 1) ? p * T(0.70710678118654752440) : p;
+}
+
+template <typename T>
+__device__ __forceinline__ cv<T> scaled(cv<T> v, T f) {
+    v.x *= f;
+    v.y *= f;
+    return v;
 }
 
 struct QftArgs {
@@ -192,7 +208,7 @@ __global__ __launch_bounds__(256) void k_qft_group(cv<T>* __restrict__ psi, uint
         }
     }
 #pragma unroll
-    for (int c = 0; c < D; ++c) stamp<T, true>(p0 + c * stride, x[c]);
+    for (int c = 0; c < D; ++c) stamp<T, true>(p0 + c * stride, scaled<T>(x[c], hscale<T>(G)));
 }
 
 // Five targets per pass with each 32-amplitude orbit split over a lane pair:
@@ -224,16 +240,15 @@ __device__ __forceinline__ void apply_d_half(cv<T>* x, uint64_t base, uint32_t t
 
 template <typename T>
 __device__ __forceinline__ void h_pair(cv<T>* x, uint32_t half) {
-    const T s = T(0.70710678118654752440);
 #pragma unroll
     for (int c = 0; c < 16; ++c) {
         const T yx = __shfl_xor(x[c].x, 1, 64), yy = __shfl_xor(x[c].y, 1, 64);
         if (half) {
-            x[c].x = (yx - x[c].x) * s;
-            x[c].y = (yy - x[c].y) * s;
+            x[c].x = yx - x[c].x;
+            x[c].y = yy - x[c].y;
         } else {
-            x[c].x = (x[c].x + yx) * s;
-            x[c].y = (x[c].y + yy) * s;
+            x[c].x = x[c].x + yx;
+            x[c].y = x[c].y + yy;
         }
     }
 }
@@ -273,7 +288,7 @@ __global__ __launch_bounds__(256) void k_qft_group5(cv<T>* __restrict__ psi, uin
         }
     }
 #pragma unroll
-    for (int c = 0; c < 16; ++c) stamp<T, true>(p0 + c * stride, x[c]);
+    for (int c = 0; c < 16; ++c) stamp<T, true>(p0 + c * stride, scaled<T>(x[c], hscale<T>(kGroupMax)));
 }
 
 // LDS position of chunk element i: bits 0..3 XOR bits 4..7, so that the
@@ -371,7 +386,8 @@ __global__ __launch_bounds__(256) void k_qft_lds(cv<T>* __restrict__ psi, uint32
                 __syncthreads();
             }
         }
-        for (uint32_t i = threadIdx.x; i < C; i += 256) stamp<T, true>(psi + cbase + i, s[swz(i)]);
+        const T hs = hscale<T>(hi - a.sq + 1);
+        for (uint32_t i = threadIdx.x; i < C; i += 256) stamp<T, true>(psi + cbase + i, scaled<T>(s[swz(i)], hs));
         __syncthreads();
     }
 }
