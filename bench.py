@@ -128,8 +128,8 @@ def gpu_dense_blocks(precision="c128", n=30, reps=3, seed=1):
     return out
 
 
-def random_circuit_c1(n, layers, seed=11):
-    """Config 1 circuit: per layer a Haar U(2) on every qubit, then a random
+def random_layered_circuit(n, layers, seed=11):
+    """Layered random circuit: per layer a Haar U(2) on every qubit, then a random
     perfect pairing with CNOT, CZ or controlled-Haar-U(2) (random orientation)."""
     from qcsim_amd import gates as G
 
@@ -170,13 +170,13 @@ def gate_bytes(gate, n, amp=16):
 
 
 def gpu_circuit(n, layers, widths=(0, 4, 5)):
-    """The config 1 random circuit, unfused and fused; one run each after a
+    """The layered random circuit, unfused and fused; one run each after a
     warm-up on a smaller circuit of the same kind."""
     from qcsim_amd import QubitRegister
 
     from qcsim_amd.tuning import apply_cache
 
-    seq = random_circuit_c1(n, layers)
+    seq = random_layered_circuit(n, layers)
     total_bytes = sum(gate_bytes(g, n) for g, _ in seq)
     reg = QubitRegister(n)
     apply_cache("c128", n)  # the sweep's launch variants, when the cache has this size
@@ -212,10 +212,11 @@ def gpu_circuit(n, layers, widths=(0, 4, 5)):
     return out
 
 
-def random_circuit_c2(n, layers, seed=13):
-    """Config 2 circuit: per layer a Haar U(2) on every qubit, CZ or CNOT on a
-    random perfect pairing, and one Toffoli-class gate (Toffoli, CCZ or
-    Fredkin) on three distinct random qubits."""
+def random_toffoli_circuit(n, layers, seed=13):
+    """Layered circuit with Toffoli-class gates: per layer a Haar U(2) on
+    every qubit, CZ or CNOT on a random perfect pairing, and one
+    Toffoli-class gate (Toffoli, CCZ or Fredkin) on three distinct random
+    qubits."""
     from qcsim_amd import gates as G
 
     rng = np.random.default_rng(seed)
@@ -231,14 +232,14 @@ def random_circuit_c2(n, layers, seed=13):
     return seq
 
 
-def gpu_precision_c2(n, layers, width=5):
+def gpu_precision_fused_c64(n, layers, width=5):
     """complex64 state running the fused circuit (FP32 MFMA blocks) beside an
     FP64 GPU copy running it unfused; error of the former against the latter."""
     from qcsim_amd import QubitRegister
 
     from qcsim_amd.tuning import apply_cache
 
-    seq = random_circuit_c2(n, layers)
+    seq = random_toffoli_circuit(n, layers)
     r64 = QubitRegister(n, precision="c128")
     r32 = QubitRegister(n, precision="c64")
     apply_cache("c64", n)  # launch variants, when the cache has this size
@@ -266,8 +267,8 @@ def gpu_precision_c2(n, layers, width=5):
     return out
 
 
-def qaoa_instances_c3(ninst, seed=31, sizes=(12, 14, 16, 18, 20, 22, 24)):
-    """Config 3 instances: n uniform over the even sizes 12..24 (a 3-regular
+def qaoa_batch_instances(ninst, seed=31, sizes=(12, 14, 16, 18, 20, 22, 24)):
+    """Batched QAOA instances: n uniform over the even sizes 12..24 (a 3-regular
     graph needs even n), J ~ N(0, 1) on edges, h ~ N(0, 0.5^2)."""
     from qcsim_amd.qaoa import random_ising
 
@@ -289,12 +290,12 @@ def qaoa_eval_bytes(n, p, tile_bits, group):
     return 16 * A * passes - 8 * A, passes
 
 
-def gpu_qaoa_c3(ninst, p=4, steps=3):
-    """Config 3: `ninst` instances in one batch, p = 4, angles uniform in
+def gpu_qaoa_batch(ninst, p=4, steps=3):
+    """Batched QAOA: `ninst` instances in one batch, p = 4, angles uniform in
     [0, pi); tile bits and group width from the autotuner."""
     from qcsim_amd.qaoa import QaoaBatch
 
-    insts, rng = qaoa_instances_c3(ninst)
+    insts, rng = qaoa_batch_instances(ninst)
     G = rng.uniform(0, np.pi, (ninst, p))
     B = rng.uniform(0, np.pi, (ninst, p))
     batch = QaoaBatch(insts, p)
@@ -329,8 +330,8 @@ def gpu_qaoa_c3(ninst, p=4, steps=3):
     }
 
 
-def gpu_learned_c4():
-    """Config 4: 1e6 (instance, angles, exact energy) rows from the batched
+def gpu_learned_qaoa():
+    """Learned QAOA proposals: 1e6 (instance, angles, exact energy) rows from the batched
     engine on 16-20 qubits, the 3x256 BF16 surrogate trained by the HIP
     kernels, top-32 proposals re-scored exactly on 8 new 20-qubit instances
     against the plain optimiser (qcsim_amd.learned.run_experiment)."""
@@ -348,9 +349,9 @@ def gpu_learned_c4():
     return out
 
 
-def cpu_qaoa_c3(p=4, seed=31):
+def cpu_qaoa_batch(p=4, seed=31):
     """The oracle's QAOA circuit (gate by gate, FP64) on one instance of each
-    config 3 size; the per-size times weighted by the batch's size mix give
+    batched-QAOA size; the per-size times weighted by the batch's size mix give
     the CPU estimate of one batch evaluation."""
     from oracle.oracle import OracleRegister, lib
     from qcsim_amd.qaoa import random_ising
@@ -407,12 +408,14 @@ def main():
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-tune", action="store_true", help="use the default launch variants")
-    ap.add_argument("--circuit-layers", type=int, default=60, help="config 1 circuit depth (0 = skip)")
-    ap.add_argument("--c2", type=int, default=0, help="run the config 2 precision leg at this many qubits (32 = full)")
-    ap.add_argument("--c2-layers", type=int, default=80)
-    ap.add_argument("--c3", type=int, default=1024, help="config 3 batched QAOA instances (0 = skip)")
-    ap.add_argument("--c4", type=int, default=1, help="config 4 learned-proposal experiment (0 = skip)")
-    ap.add_argument("--only", choices=["qft", "circuit", "dense", "c2", "c3", "c4"],
+    ap.add_argument("--circuit-layers", type=int, default=60,
+                    help="depth of the 30-qubit layered random circuit leg (0 = skip)")
+    ap.add_argument("--precision-qubits", type=int, default=0,
+                    help="run the complex64-fused versus FP64 leg at this many qubits (32 = full size)")
+    ap.add_argument("--precision-layers", type=int, default=80)
+    ap.add_argument("--qaoa-instances", type=int, default=1024, help="batched QAOA instances (0 = skip)")
+    ap.add_argument("--learned", type=int, default=1, help="learned-proposal QAOA experiment (0 = skip)")
+    ap.add_argument("--only", choices=["qft", "circuit", "dense", "precision", "qaoa", "learned"],
                     help="run one leg alone and print its JSON (for profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -428,9 +431,9 @@ def main():
             "qft": lambda: gpu_qft(n),
             "circuit": lambda: gpu_circuit(n, args.circuit_layers),
             "dense": lambda: [gpu_dense_blocks("c128", n), gpu_dense_blocks("c64", n + 2)],
-            "c2": lambda: gpu_precision_c2(args.c2 or 32, args.c2_layers),
-            "c3": lambda: gpu_qaoa_c3(args.c3),
-            "c4": gpu_learned_c4,
+            "precision": lambda: gpu_precision_fused_c64(args.precision_qubits or 32, args.precision_layers),
+            "qaoa": lambda: gpu_qaoa_batch(args.qaoa_instances),
+            "learned": gpu_learned_qaoa,
         }
         print(json.dumps({"leg": args.only, "result": legs[args.only]()}))
         return
@@ -445,22 +448,22 @@ def main():
     progress(f"qft: {qft['ms']} ms")
     circuit = gpu_circuit(n, args.circuit_layers) if args.circuit_layers > 0 else None
     progress("circuit leg done")
-    c2 = gpu_precision_c2(args.c2, args.c2_layers) if args.c2 else None
-    progress("config 2 leg done")
-    c3 = gpu_qaoa_c3(args.c3) if args.c3 else None
-    progress("config 3 leg done")
-    c4 = gpu_learned_c4() if args.c4 else None
-    progress("config 4 leg done")
+    prec = gpu_precision_fused_c64(args.precision_qubits, args.precision_layers) if args.precision_qubits else None
+    progress("precision leg done")
+    qaoa = gpu_qaoa_batch(args.qaoa_instances) if args.qaoa_instances else None
+    progress("batched QAOA leg done")
+    learned = gpu_learned_qaoa() if args.learned else None
+    progress("learned QAOA leg done")
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(n, [0, 7, 15, 22, 29] if n >= 30 else list(range(0, n, max(1, n // 5))))
         progress("cpu baseline done")
-        if c3:
-            per_n, cores = cpu_qaoa_c3()
-            est = sum(per_n[k] * v for k, v in c3["qubits_count"].items())
-            c3["cpu_baseline"] = {
+        if qaoa:
+            per_n, cores = cpu_qaoa_batch()
+            est = sum(per_n[k] * v for k, v in qaoa["qubits_count"].items())
+            qaoa["cpu_baseline"] = {
                 "batch_s_estimate": round(est, 2),
-                "evals_per_s": round(c3["instances"] / est, 2),
+                "evals_per_s": round(qaoa["instances"] / est, 2),
                 "cores": cores,
                 "kind": "port",
                 "sample": "one instance per size, gate-by-gate FP64 circuit plus energy: "
@@ -497,9 +500,9 @@ def main():
         "launch_variants": tuned or "default",
         "qft": qft,
         "circuit": circuit,
-        "precision_c2": c2,
-        "qaoa_c3": c3,
-        "learned_c4": c4,
+        "precision_c64": prec,
+        "qaoa_batch": qaoa,
+        "learned_qaoa": learned,
         "host_wall_s": round(wall, 4),
         "norm_drift": norm_err,
     }

@@ -197,7 +197,7 @@ def compare_on_instance(model: Surrogate, inst: IsingInstance, p: int, rng: np.r
 
 def run_experiment(train_rows=1_000_000, train_instances=2048, sizes=(16, 18, 20), p=4, epochs=20, test_n=20,
                    test_instances=8, seed=2024, top=32, progress=None):
-    """Config 4 end to end: generate the data set with the batched engine,
+    """The learned-proposal experiment end to end: generate the data set with the batched engine,
     train the surrogate, compare learned proposals with the plain optimiser
     on new instances (compare_on_instance). Returns a JSON-ready summary."""
     rng = np.random.default_rng(seed)

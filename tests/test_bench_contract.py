@@ -38,13 +38,13 @@ def test_algorithmic_bytes_per_gate_family():
 
 
 def test_workload_generators_are_deterministic():
-    a = bench.random_circuit_c1(8, 3)
-    b = bench.random_circuit_c1(8, 3)
+    a = bench.random_layered_circuit(8, 3)
+    b = bench.random_layered_circuit(8, 3)
     assert len(a) == 3 * (8 + 4)
     for (ga, qa), (gb, qb) in zip(a, b):
         assert qa == qb
         assert (ga.getRawOperatorMatrix() == gb.getRawOperatorMatrix()).all()
-    c2 = bench.random_circuit_c2(8, 2)
+    c2 = bench.random_toffoli_circuit(8, 2)
     assert len(c2) == 2 * (8 + 4 + 1)
     assert sum(1 for g, _ in c2 if g.getQubitsNumber() == 3) == 2
 

@@ -265,7 +265,7 @@ def test_ising_energy_matches_oracle():
     assert abs(g.ising_energy(edges, J, h) - o.ising_energy(edges, J, h)) < 1e-12
 
 
-def test_random_circuit_config0():
+def test_random_circuit_20_qubits_with_qft_and_samples():
     """20 qubits complex128: 40 layers of Haar U(2) on every qubit + CNOT/CZ
     on a random pairing, then a 20-qubit QFT; 1024 supplied uniforms."""
     n, layers = 20, 40
@@ -288,14 +288,14 @@ def test_random_circuit_config0():
     got = g.getRegisterStorage()
     err = float(np.max(np.abs(got - ref)))
     infid = 1 - abs(np.vdot(ref, got)) ** 2
-    print(f"\nconfig0: max-abs {err:.3e} infidelity {infid:.3e}")
+    print(f"\n20-qubit circuit: max-abs {err:.3e} infidelity {infid:.3e}")
     assert err < TOL64
     us = rng.random(1024)
     margin = cdf_margin(ref, 1 - us)
     sg = g.sample(us)
     so = o.sample(us)
     bad = np.nonzero(sg != so)[0]
-    print(f"config0: {len(bad)} sample mismatches, min CDF margin {margin.min():.3e}")
+    print(f"20-qubit circuit: {len(bad)} sample mismatches, min CDF margin {margin.min():.3e}")
     assert all(margin[i] < 1e-10 for i in bad)
 
 
@@ -397,13 +397,13 @@ def test_fused_circuit(precision, tol, width):
     assert err < tol, err
 
 
-def test_precision_config2_structure():
-    """The config 2 circuit structure at 20 qubits: the complex64 register
+def test_precision_toffoli_circuit_c64_against_fp64():
+    """The layered circuit with Toffoli-class gates at 20 qubits: the complex64 register
     running fused FP32 MFMA blocks stays within 1e-5 max-abs and 1e-8
     infidelity of the FP64 register running the same gates unfused."""
     import bench
 
-    out = bench.gpu_precision_c2(20, 20)
+    out = bench.gpu_precision_fused_c64(20, 20)
     print(out)
     assert out["max_abs_error"] < 1e-5
     assert out["infidelity"] < 1e-8
@@ -429,7 +429,7 @@ def test_clone_survives_overwrite_of_source():
 
 
 def test_fused_matches_unfused_long_circuit():
-    """The config 1 circuit at 24 qubits (a long queue of launches on the
+    """The layered random circuit at 24 qubits (a long queue of launches on the
     register's stream): fused widths 4 and 5 agree with the unfused run."""
     import bench
 
@@ -440,15 +440,15 @@ def test_fused_matches_unfused_long_circuit():
         assert out[k]["infidelity_vs_unfused"] < 1e-12, out[k]
 
 
-def test_config1_circuit_against_the_oracle_24_qubits():
-    """The config 1 circuit (60 layers: Haar U(2) on every qubit, then CNOT /
+def test_layered_circuit_against_the_oracle_24_qubits():
+    """The layered random circuit (60 layers: Haar U(2) on every qubit, then CNOT /
     CZ / controlled-Haar-U(2) on a random pairing) at 24 qubits, complex128,
     unfused and fused to widths 4 and 5 (MFMA blocks), against the oracle's
     gate-by-gate run from |0...0>: max-abs <= 1e-12, infidelity printed."""
     import bench
 
     n = 24
-    seq = bench.random_circuit_c1(n, 60)
+    seq = bench.random_layered_circuit(n, 60)
     o = OracleRegister(n)
     for gate, qs in seq:
         o.apply_gate(gate, *qs)
