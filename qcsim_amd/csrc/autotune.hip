@@ -32,7 +32,7 @@ static const Variant kVariants[] = {{2, 1, 1}, {2, 0, 1}, {1, 0, 0}, {1, 1, 0}, 
                                     {2, 1, 1, 20}, {4, 1, 1, 16}, {2, 3, 1, 16}, {2, 1, 1, 14},
                                     {2, 1, 1, 18}, {4, 1, 1, 12}, {2, 1, 2, 16}, {4, 1, 1, 14},
                                     {1, 1, 1, 0, 1}, {1, 1, 0, 0, 1}, {1, 0, 1, 0, 1}, {1, 3, 1, 0, 1},
-                                    {1, 1, 2, 0, 1}};
+                                    {1, 1, 2, 0, 1}, {1, 1, 1, 0, 2}, {1, 1, 0, 0, 2}, {1, 3, 1, 0, 2}};
 constexpr int kNumVariants = int(sizeof(kVariants) / sizeof(kVariants[0]));
 constexpr int kMaxN = 40;
 

@@ -170,7 +170,7 @@ int stage_upload(qcs_register* r, const void* src, size_t bytes, const void** de
 struct Variant {
     uint8_t p, nt, xcd;
     uint8_t sp = 0;  // k_1q only: log2 distance of a thread's orbits (0: kThreads)
-    uint8_t x = 0;   // k_1q only: 1 = one amplitude per lane, partner through LDS
+    uint8_t x = 0;   // k_1q only: one amplitude per lane, partner 1: through LDS, 2: by lane shuffle
 };
 Variant pick_variant(const qcs_register* r, uint32_t lowbit, bool k1q = false);
 int variant_count();

@@ -151,6 +151,50 @@ __global__ __launch_bounds__(kThreads) void k_1q_split(cv<T>* __restrict__ psi, 
     stamp<T, NT == 1 || NT == 3>(psi + idx, u);
 }
 
+// k_1q_split with the partner in the same wave, exchanged by a lane shuffle
+// (no LDS, no barrier): for tl < 6 a wave takes 64 consecutive values of e
+// (partner lane ^ 2^tl); otherwise 32 consecutive orbits, lanes 0..31 with
+// the target bit clear and lanes 32..63 with it set (partner lane ^ 32).
+template <typename T, int MODE, int NT, int XCD>
+__global__ __launch_bounds__(kThreads) void k_1q_shfl(cv<T>* __restrict__ psi, Ins insc, uint64_t cmask,
+                                                      uint32_t tl, MatD<T, 2> m, uint64_t boff) {
+    const uint64_t bb = block_id<XCD>() + boff;
+    const int tid = threadIdx.x, lane = tid & 63;
+    uint64_t e;
+    int h, mask;
+    if (tl < 6) {
+        e = bb * kThreads + tid;
+        h = (lane >> tl) & 1;
+        mask = 1 << tl;
+    } else {
+        const uint64_t o = (bb * (kThreads / 64) + (tid >> 6)) * 32 + (lane & 31);
+        const uint64_t low = (1ull << tl) - 1;
+        h = lane >> 5;
+        e = ((o & ~low) << 1) | (o & low) | (uint64_t(h) << tl);
+        mask = 32;
+    }
+    const uint64_t idx = insert_zeros(e, insc) | cmask;
+    const cv<T> a = ldamp<T, NT == 1 || NT == 2>(psi + idx);
+    cv<T> u;
+    if (MODE == 2) {
+        u = h ? cmul<T>(m.re[3], m.im[3], a) : cmul<T>(m.re[0], m.im[0], a);
+    } else {
+        cv<T> b;
+        b.x = __shfl_xor(a.x, mask, 64);
+        b.y = __shfl_xor(a.y, mask, 64);
+        const cv<T> x0 = h ? b : a, x1 = h ? a : b;
+        if (MODE == 0) {
+            const T r0 = h ? m.re[2] : m.re[0], i0 = h ? m.im[2] : m.im[0];
+            const T r1 = h ? m.re[3] : m.re[1], i1 = h ? m.im[3] : m.im[1];
+            u.x = r0 * x0.x - i0 * x0.y + r1 * x1.x - i1 * x1.y;
+            u.y = r0 * x0.y + i0 * x0.x + r1 * x1.y + i1 * x1.x;
+        } else {
+            u = h ? cmul<T>(m.re[2], m.im[2], x0) : cmul<T>(m.re[1], m.im[1], x1);
+        }
+    }
+    stamp<T, NT == 1 || NT == 3>(psi + idx, u);
+}
+
 // Phase on the single amplitude of each orbit whose listed bits are all 1:
 // P/S/T/Z (m00 == 1), CZ/CP/CRz-with-unit-m22 (K6), CCZ (K11).
 template <typename T, int P, int NT, int XCD>
@@ -410,6 +454,32 @@ static int k1q_t(qcs_register* r, int mode, uint32_t target, const uint32_t* ctr
         else QCS_L1S(1, 8);
 #undef QCS_L1S
         return check_launch(r, "k_1q_split");
+    }
+    if (v.x == 2 && norb >= kThreads / 2) {
+        // one amplitude per lane, partner by lane shuffle (k_1q_shfl)
+        uint32_t tl = target;
+        for (int i = 0; i < nctrl; ++i) tl -= ctrls[i] < target ? 1 : 0;
+        const Ins insc = make_ins(ctrls, nctrl);
+        const uint64_t nblk = norb / (kThreads / 2);
+#define QCS_L1W(NTT, XX)                                                                                       \
+    do {                                                                                                       \
+        QCS_CHUNKED(nblk, {                                                                                    \
+            if (mode == 0)                                                                                     \
+                hipLaunchKernelGGL((k_1q_shfl<T, 0, NTT, XX>), dim3(g), dim3(kThreads), 0, r->stream, psi, insc, \
+                                   cmask, tl, mm, boff);                                                       \
+            else if (mode == 1)                                                                                \
+                hipLaunchKernelGGL((k_1q_shfl<T, 1, NTT, XX>), dim3(g), dim3(kThreads), 0, r->stream, psi, insc, \
+                                   cmask, tl, mm, boff);                                                       \
+            else                                                                                               \
+                hipLaunchKernelGGL((k_1q_shfl<T, 2, NTT, XX>), dim3(g), dim3(kThreads), 0, r->stream, psi, insc, \
+                                   cmask, tl, mm, boff);                                                       \
+        })                                                                                                     \
+    } while (0)
+        if (v.nt == 1 && v.xcd == 0) QCS_L1W(1, 0);
+        else if (v.nt == 3 && v.xcd == 1) QCS_L1W(3, 8);
+        else QCS_L1W(1, 8);
+#undef QCS_L1W
+        return check_launch(r, "k_1q_shfl");
     }
     QCS_VARIANT_SWITCH(v, QCS_L1Q);
 #undef QCS_L1Q
