@@ -23,13 +23,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per gate launch (k_1q_split) on the 30-qubit complex128 state,
+# HBM bytes per gate launch (the tuned 1-qubit kernels) on the 30-qubit complex128 state,
 # measured with rocprofv3 --pmc FETCH_SIZE and, in a separate run, WRITE_SIZE
-# (profiles/k_1q_sweep30_c128.md): 16,777,250 KiB fetched and 16,779,848 KiB written
+# (profiles/k_1q_sweep30_c128.md): 16,777,248 KiB fetched and 16,780,554 KiB written
 # per dispatch (means over the 30 targets) after calibration against kernels
 # of known traffic (a full norm reads 16 GiB: FETCH_SIZE x2.000; the state
 # fill writes 16 GiB: WRITE_SIZE x1.000).
-K1Q_30Q_TRAFFIC_BYTES = (16_777_250 + 16_779_848) * 1024
+K1Q_30Q_TRAFFIC_BYTES = (16_777_248 + 16_780_554) * 1024
 
 
 def algorithmic_bytes_1q(n, amp_bytes=16):
