@@ -13,6 +13,7 @@
 // k_1q_split is the exception: there a block owns 128 whole orbits and each
 // lane one amplitude of them, exchanged through LDS.
 #include <algorithm>
+#include <vector>
 
 #include "common.h"
 
@@ -274,6 +275,60 @@ __global__ __launch_bounds__(kThreads) void k_2q(cv<T>* __restrict__ psi, uint64
         y.y = yi;
         psi[idx[r]] = y;
     }
+}
+
+// Generic 2^K x 2^K update (K = 2: K9 and K14; K = 3: K15) with one
+// amplitude per lane. A block owns a tile of 256 amplitudes of the
+// control-stripped index space: the K target bits plus the 8 - K lowest
+// other bits (tile positions, ascending, so consecutive lanes hit
+// consecutive addresses wherever those positions run from bit 0). Each
+// lane puts its amplitude in LDS, gathers the 2^K amplitudes of its orbit
+// and computes its own row of the matrix (staged in LDS as Re then Im).
+struct NqGeom {
+    uint8_t pos[8];    // tile positions in the stripped index, ascending
+    uint8_t tslot[3];  // lane-index bit that holds target i
+};
+
+template <typename T, int K>
+__global__ __launch_bounds__(kThreads) void k_nq_split(cv<T>* __restrict__ psi, Ins insc, uint64_t cmask, NqGeom gm,
+                                                       const T* __restrict__ mat, uint64_t boff) {
+    constexpr int D = 1 << K;
+    __shared__ cv<T> s[kThreads];
+    __shared__ T sm[2 * D * D];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < 2 * D * D; i += kThreads) sm[i] = mat[i];
+    uint64_t e = block_id<8>() + boff;
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+        const uint64_t low = (1ull << gm.pos[p]) - 1;
+        e = ((e & ~low) << 1) | (e & low);
+    }
+#pragma unroll
+    for (int p = 0; p < 8; ++p) e |= uint64_t((tid >> p) & 1) << gm.pos[p];
+    const uint64_t idx = insert_zeros(e, insc) | cmask;
+    s[tid] = ldamp<T, true>(psi + idx);
+    int c = 0, tmask = 0;
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+        c |= ((tid >> gm.tslot[i]) & 1) << i;
+        tmask |= 1 << gm.tslot[i];
+    }
+    __syncthreads();
+    T yr = 0, yi = 0;
+#pragma unroll
+    for (int cp = 0; cp < D; ++cp) {
+        int lane = tid & ~tmask;
+#pragma unroll
+        for (int i = 0; i < K; ++i) lane |= ((cp >> i) & 1) << gm.tslot[i];
+        const cv<T> x = s[lane];
+        const T mr = sm[c * D + cp], mi = sm[D * D + c * D + cp];
+        yr += mr * x.x - mi * x.y;
+        yi += mr * x.y + mi * x.x;
+    }
+    cv<T> y;
+    y.x = yr;
+    y.y = yi;
+    stamp<T, true>(psi + idx, y);
 }
 
 // Generic 8x8 (K15): local bits (b0, b1, b2) = (qubit, c1, c2).
@@ -562,9 +617,64 @@ int kswap(qcs_register* r, uint32_t a, uint32_t b, const uint32_t* ctrls, int nc
                                      : kswap_t<float>(r, a, b, ctrls, nctrl, ph, phased);
 }
 
+// k_nq_split launch for targets t[0..K) (qubit order = matrix bit order) and
+// controls. Sets *launched = false when the state is too small for a tile
+// (the caller then uses the orbit kernel).
+template <typename T, int K>
+static int knq_split(qcs_register* r, const uint32_t* t, const uint32_t* ctrls, int nctrl, const hcplx* m,
+                     bool* launched) {
+    *launched = false;
+    const uint32_t nstr = r->n - uint32_t(nctrl);
+    if (nstr < 8) return QCS_OK;
+    constexpr int D = 1 << K;
+    uint64_t cmask = 0;
+    uint32_t ts[K];
+    for (int i = 0; i < nctrl; ++i) cmask |= 1ull << ctrls[i];
+    for (int i = 0; i < K; ++i) {
+        ts[i] = t[i];
+        for (int j = 0; j < nctrl; ++j) ts[i] -= ctrls[j] < t[i] ? 1 : 0;
+    }
+    uint32_t all[8];
+    int na = 0;
+    for (int i = 0; i < K; ++i) all[na++] = ts[i];
+    for (uint32_t b = 0; na < 8; ++b) {
+        bool tgt = false;
+        for (int i = 0; i < K; ++i) tgt = tgt || ts[i] == b;
+        if (!tgt) all[na++] = b;
+    }
+    std::sort(all, all + 8);
+    NqGeom gm{};
+    for (int p = 0; p < 8; ++p) gm.pos[p] = uint8_t(all[p]);
+    for (int i = 0; i < K; ++i)
+        for (int p = 0; p < 8; ++p)
+            if (all[p] == ts[i]) gm.tslot[i] = uint8_t(p);
+    std::vector<T> hm(2 * D * D);
+    for (int i = 0; i < D * D; ++i) {
+        hm[i] = T(m[i].re);
+        hm[D * D + i] = T(m[i].im);
+    }
+    const void* dmat = nullptr;
+    if (const int e = stage_upload(r, hm.data(), sizeof(T) * hm.size(), &dmat)) return e;
+    const Ins insc = make_ins(ctrls, nctrl);
+    const uint64_t nblk = (uint64_t(1) << nstr) / kThreads;
+    QCS_CHUNKED(nblk, hipLaunchKernelGGL((k_nq_split<T, K>), dim3(g), dim3(kThreads), 0, r->stream,
+                                         static_cast<cv<T>*>(r->psi), insc, cmask, gm, static_cast<const T*>(dmat),
+                                         boff));
+    *launched = true;
+    return check_launch(r, "k_nq_split");
+}
+
 template <typename T>
 static int k2q_t(qcs_register* r, uint32_t t0, uint32_t t1, const uint32_t* ctrls, int nctrl,
                  const hcplx* m) {
+    {
+        // measured on MI355X at 30 qubits (complex128): 6.1-6.3 ms against
+        // 6.5-7.2 ms for the orbit kernel below
+        const uint32_t t[2] = {t0, t1};
+        bool launched = false;
+        const int e = knq_split<T, 2>(r, t, ctrls, nctrl, m, &launched);
+        if (e || launched) return e;
+    }
     uint32_t pos[6] = {t0, t1};
     uint64_t cmask = 0;
     for (int i = 0; i < nctrl; ++i) {
@@ -592,6 +702,17 @@ int k2q(qcs_register* r, uint32_t t0, uint32_t t1, const uint32_t* ctrls, int nc
 
 template <typename T>
 static int k3q_t(qcs_register* r, uint32_t t0, uint32_t t1, uint32_t t2, const hcplx* m) {
+    {
+        // measured on MI355X at 30 qubits (complex128): the lane kernel wins
+        // when a target is below bit 3 (7.2 against 8.7 ms for qubits 0-2),
+        // the orbit kernel otherwise (6.4-6.5 against 6.6-6.8 ms)
+        const uint32_t t[3] = {t0, t1, t2};
+        bool launched = false;
+        if (std::min(t0, std::min(t1, t2)) < 3) {
+            const int e = knq_split<T, 3>(r, t, nullptr, 0, m, &launched);
+            if (e || launched) return e;
+        }
+    }
     uint32_t pos[3] = {t0, t1, t2};
     const Ins ins = make_ins(pos, 3);
     const uint64_t norb = r->A >> 3;
