@@ -486,7 +486,7 @@ def main():
             "workload": f"{n}-qubit complex128 statevector, Haar U(2) sweep over targets 0..{n - 1} (one step = {n} gates)",
             "qubits": n,
             "state_bytes": (1 << n) * 16,
-            "kernel": "k_1q_split<double,0,NT,XCD> / k_1q<double,0,P,NT,XCD> (generic 1-qubit, in place; launch variant per target from the autotuner)",
+            "kernel": "k_1q_split<double,0,NT,XCD> / k_1q_shfl<double,0,NT,XCD> / k_1q<double,0,P,NT,XCD> (generic 1-qubit, in place; kernel and launch variant per target from the autotuner)",
         },
         "roofline": {
             "bound": "hbm",
