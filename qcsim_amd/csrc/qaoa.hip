@@ -289,10 +289,11 @@ __device__ void thread_terms(ThreadTerms& W, double gamma, const QInst& I, const
 // 2^tb amplitudes, 8 <= tb = 8 + M <= 12, whose element 0 has global index
 // `base`; W and TC.q come from thread_terms<M> for this instance and gamma.
 // Thread t takes elements t + 256 m (m < 2^M: the tile bits 8..tb-1) in Gray
-// order over m, applies the cost phase and stores them in the (swizzled) LDS
-// tile; then three rounds apply Rx on qubits 8..tb-1, 4..7 and 0..3 with
-// orbits held in registers. The input is |+> (amplitude amp), `in` (global)
-// or the LDS tile; the result is left in the LDS tile.
+// order over m and applies the cost phase; these 2^M elements are the orbit
+// of qubits 8..tb-1, so their Rx follows in registers before the elements go
+// to the (swizzled) LDS tile; two rounds then apply Rx on qubits 4..7 and
+// 0..3 with orbits held in registers. The input is |+> (amplitude amp), `in`
+// (global) or the LDS tile; the result is left in the LDS tile.
 //
 // The cost phase splits along the tile: with z_i = 1 - 2 b_i,
 //   C = K + A(t) + sum_{i in 8..tb-1} z_i F'_i(t) + Q(m),
@@ -301,8 +302,11 @@ __device__ void thread_terms(ThreadTerms& W, double gamma, const QInst& I, const
 //   exp(-i gamma C) = exp(-i gamma (K + A + sum F')) prod_{b_i = 1}
 //                     exp(2 i gamma F'_i) q[m],
 // with exp(2 i gamma F'_i) = ef[i-8] eg[i-8]: per tile one sincos per thread,
-// and per amplitude one FP64 complex product along the Gray walk and two FP32
-// ones, instead of a sincos and an energy update per amplitude.
+// and per amplitude three FP32 complex products (one along the Gray walk),
+// instead of a sincos and an energy update per amplitude. The walk runs in
+// FP32 so that the orbit, the walk and the Rx fit the 128 VGPRs that four
+// waves per SIMD allow; its 2^M - 1 steps add at most about 2^M FP32
+// roundings to a phase that is itself an FP32 sincos.
 //
 // `tid` is threadIdx.x passed through an empty asm statement by the caller,
 // so that the compiler recomputes the LDS addresses per tile instead of
@@ -334,39 +338,43 @@ __device__ void tile_layer(float2* t, uint32_t tid, uint64_t base, double gamma,
     double A = TC.K + W.c0;
 #pragma unroll
     for (uint32_t i = 0; i < 8; ++i) A += zf(tid, i) * TC.F[i];
-    double2 e[M > 0 ? M : 1];
+    float2 e[M > 0 ? M : 1];
 #pragma unroll
     for (int k = 0; k < M; ++k) {
         A += TC.F[8 + k];
-        // exp(2 i gamma F'_k) = ef eg in FP64, rescaled to modulus 1 (to
-        // O(delta^2) for |.|^2 = 1 + delta) so that the walk's alternating
-        // products by it and its conjugate cancel
+        // exp(2 i gamma F'_k) = ef eg, multiplied and rescaled to modulus 1
+        // (to O(delta^2) for |.|^2 = 1 + delta) in FP64, then rounded, so
+        // that the walk's alternating products by it and its conjugate cancel
+        // to FP32 rounding
         const float2 a = TC.ef[k], b = W.eg[k];
         const double x = double(a.x) * b.x - double(a.y) * b.y, y = double(a.x) * b.y + double(a.y) * b.x;
         const double r = 1.5 - 0.5 * (x * x + y * y);
-        e[k] = make_double2(x * r, y * r);
+        e[k] = make_float2(float(x * r), float(y * r));
     }
-    const float2 p0 = expmi(gamma * A);
-    double2 ph = make_double2(p0.x, p0.y);
+    float2 ph = expmi(gamma * A);
+    float2 y[1 << M];
 #pragma unroll
     for (uint32_t m = 0; m < (1u << M); ++m) {
         const uint32_t gm = m ^ (m >> 1);
         if (m) {
             const int k = __builtin_ctz(m);
-            const double2 f = e[k];
-            const double fy = ((gm >> k) & 1) ? f.y : -f.y;  // set: times e_k; cleared: divided by it
-            ph = make_double2(ph.x * f.x - ph.y * fy, ph.x * fy + ph.y * f.x);
+            const float2 f = e[k];
+            const float fy = ((gm >> k) & 1) ? f.y : -f.y;  // set: times e_k; cleared: divided by it
+            ph = make_float2(ph.x * f.x - ph.y * fy, ph.x * fy + ph.y * f.x);
         }
         const float2 qm = TC.q[gm];
-        const float pr = float(ph.x), pi = float(ph.y);
+        const float pr = ph.x, pi = ph.y;
         const float wr = pr * qm.x - pi * qm.y, wi = pr * qm.y + pi * qm.x;
         const uint32_t i = tid + (gm << 8);
         const float2 v = in_mode == IN_GLOBAL ? ldamp<float, true>(in + i)
                                               : (in_mode == IN_LDS ? t[swz(i)] : make_float2(amp, 0.f));
-        t[swz(i)] = make_float2(wr * v.x - wi * v.y, wr * v.y + wi * v.x);
+        y[gm] = make_float2(wr * v.x - wi * v.y, wr * v.y + wi * v.x);
     }
+    // y is the orbit of qubits 8..tb-1: their Rx before the LDS store
+    if constexpr (M > 0) rx_reg<M>(y, c, s);
+#pragma unroll
+    for (uint32_t m = 0; m < (1u << M); ++m) t[swz(tid + (m << 8))] = y[m];
     __syncthreads();
-    if (M > 0) rx_round<(M > 0 ? M : 1)>(t, tid, tb, 8, c, s);
     rx_round<4>(t, tid, tb, 4, c, s);
     rx_round<4>(t, tid, tb, 0, c, s);
 }

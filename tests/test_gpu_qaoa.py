@@ -20,7 +20,7 @@ def make_batch_instances(seed, sizes):
     return [random_ising(n, rng) for n in sizes]
 
 
-@pytest.mark.parametrize("tile_bits,group", [(12, 6), (10, 4), (8, 5), (12, 4), (11, 5)])
+@pytest.mark.parametrize("tile_bits,group", [(12, 6), (10, 4), (8, 5), (12, 4), (11, 5), (6, 4)])
 def test_batch_matches_oracle(tile_bits, group):
     sizes = [4, 6, 8, 10, 12, 14, 16, 18]
     insts = make_batch_instances(tile_bits * 7 + group, sizes)
