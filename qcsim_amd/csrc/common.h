@@ -151,6 +151,10 @@ struct qcs_register {
 
 namespace qcs {
 
+// A launch covers at most kMaxBlocks blocks of 256 (2^30 work-items: a
+// grid's size in work-items must stay below 2^32); larger one-shot grids
+// (complex64 states of 33 qubits and up) run as several launches.
+constexpr uint64_t kMaxBlocks = 1ull << 22;
 constexpr int kRedBlocks = 4096;     // fixed grid of reductions (deterministic)
 constexpr int kRedThreads = 256;
 constexpr uint32_t kChunkLog = 11;   // CDF chunk = 2048 amplitudes

@@ -35,10 +35,8 @@ static uint64_t grid_for(uint64_t norb, int per_block) {
     return (norb + per_block - 1) / per_block;
 }
 
-// A launch covers at most kMaxBlocks blocks (2^30 work-items: a grid's size
-// in work-items must stay below 2^32). Larger sweeps (complex64 states of
-// 33 qubits and up) run as several launches, each told its first block.
-constexpr uint64_t kMaxBlocks = 1ull << 22;
+// Larger sweeps than kMaxBlocks blocks run as several launches, each told
+// its first block.
 #define QCS_CHUNKED(NBLK, LAUNCH1)                                                    \
     for (uint64_t boff = 0; boff < (NBLK); boff += kMaxBlocks) {                      \
         const unsigned g = unsigned(std::min<uint64_t>(kMaxBlocks, (NBLK) - boff));   \

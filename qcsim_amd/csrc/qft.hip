@@ -340,56 +340,49 @@ __device__ void lds_round_g(cv<T>* s, uint64_t cbase, uint32_t clog, const QftAr
 // Targets sq..hi (hi < clog <= kLdsLog) inside contiguous chunks of 2^clog
 // amplitudes staged in LDS, in rounds of up to kLdsRound consecutive targets
 // (from the top down for the QFT, from sq up for its inverse); one LDS round
-// trip per round instead of one per target.
+// trip per round instead of one per target. One chunk per block (one-shot
+// grid, in launches of at most kMaxBlocks blocks from chunk boff).
 template <typename T, bool INV>
-__global__ __launch_bounds__(256) void k_qft_lds(cv<T>* __restrict__ psi, uint32_t clog, uint64_t nchunks, QftArgs a,
+__global__ __launch_bounds__(256) void k_qft_lds(cv<T>* __restrict__ psi, uint32_t clog, uint64_t boff, QftArgs a,
                                                  uint32_t hi, const double2* __restrict__ tw, const QftRoots R) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     cv<T>* s = reinterpret_cast<cv<T>*>(smem_raw);
     constexpr int EPT = (1 << kLdsLog) / 256;  // chunk elements per thread
     const uint32_t C = 1u << clog;
-    // The blocks stride over the chunks; each loads its next chunk into
-    // registers while the current one is transformed in LDS.
-    cv<T> pre[EPT];
-    uint64_t chunk = blockIdx.x;
-    auto load = [&](uint64_t ch) {
+    const uint64_t cbase = (boff + blockIdx.x) << clog;
+    {
+        cv<T> v[EPT];
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
             const uint32_t i = threadIdx.x + 256u * k;
-            if (i < C) pre[k] = ldamp<T, kStagedNtLoads>(psi + (ch << clog) + i);
+            if (i < C) v[k] = ldamp<T, kStagedNtLoads>(psi + cbase + i);
         }
-    };
-    if (chunk < nchunks) load(chunk);
-    for (; chunk < nchunks; chunk += gridDim.x) {
-        const uint64_t cbase = chunk << clog;
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
             const uint32_t i = threadIdx.x + 256u * k;
-            if (i < C) s[swz(i)] = pre[k];
+            if (i < C) s[swz(i)] = v[k];
         }
-        __syncthreads();
-        if (chunk + gridDim.x < nchunks) load(chunk + gridDim.x);
-        if (!INV) {
-            for (int top = int(hi); top >= int(a.sq); top -= kLdsRound) {
-                QftArgs r = a;
-                r.lo = uint32_t(top - (kLdsRound - 1) > int(a.sq) ? top - (kLdsRound - 1) : int(a.sq));
-                r.g = uint32_t(top) - r.lo + 1;
-                lds_round_g<T, false>(s, cbase, clog, r, tw, R);
-                __syncthreads();
-            }
-        } else {
-            for (uint32_t lo = a.sq; lo <= hi; lo += kLdsRound) {
-                QftArgs r = a;
-                r.lo = lo;
-                r.g = hi - lo + 1 < uint32_t(kLdsRound) ? hi - lo + 1 : uint32_t(kLdsRound);
-                lds_round_g<T, true>(s, cbase, clog, r, tw, R);
-                __syncthreads();
-            }
-        }
-        const T hs = hscale<T>(hi - a.sq + 1);
-        for (uint32_t i = threadIdx.x; i < C; i += 256) stamp<T, true>(psi + cbase + i, scaled<T>(s[swz(i)], hs));
         __syncthreads();
     }
+    if (!INV) {
+        for (int top = int(hi); top >= int(a.sq); top -= kLdsRound) {
+            QftArgs r = a;
+            r.lo = uint32_t(top - (kLdsRound - 1) > int(a.sq) ? top - (kLdsRound - 1) : int(a.sq));
+            r.g = uint32_t(top) - r.lo + 1;
+            lds_round_g<T, false>(s, cbase, clog, r, tw, R);
+            __syncthreads();
+        }
+    } else {
+        for (uint32_t lo = a.sq; lo <= hi; lo += kLdsRound) {
+            QftArgs r = a;
+            r.lo = lo;
+            r.g = hi - lo + 1 < uint32_t(kLdsRound) ? hi - lo + 1 : uint32_t(kLdsRound);
+            lds_round_g<T, true>(s, cbase, clog, r, tw, R);
+            __syncthreads();
+        }
+    }
+    const T hs = hscale<T>(hi - a.sq + 1);
+    for (uint32_t i = threadIdx.x; i < C; i += 256) stamp<T, true>(psi + cbase + i, scaled<T>(s[swz(i)], hs));
 }
 
 // ---- bit reversal of the range [sq, eq] ---------------------------------------
@@ -423,76 +416,75 @@ struct BitrevArgs {
     uint32_t sq, m, b;     // range start, range length, tile bits
     uint32_t rowlog;       // sq + b
     uint32_t midlo, midn;  // Mid bits: [sq+b, sq+b+midn)
+    uint32_t hh;           // midn / 2
 };
 
+// Tile pairs per value of the bits above eq: with Mid = [A | c | B] (A, B of
+// hh bits, c the middle bit when midn is odd), rev(Mid) = [rev B | c | rev A],
+// so each unordered pair is exactly one (A, c, B) with A <= x = rev(B): for
+// each c, sum over x of (x + 1) = 2^hh (2^hh + 1) / 2 pairs.
+__host__ __device__ __forceinline__ uint64_t bitrev_pairs_per_outer(const BitrevArgs& a) {
+    return (((uint64_t(1) << a.hh) * ((uint64_t(1) << a.hh) + 1)) / 2) << (a.midn & 1);
+}
+
+// One block per tile pair (one-shot grid, in launches of at most kMaxBlocks
+// blocks from block boff).
 template <typename T>
-__global__ __launch_bounds__(256) void k_bitrev_tile(cv<T>* __restrict__ psi, uint64_t ntiles, BitrevArgs a) {
+__global__ __launch_bounds__(256) void k_bitrev_tile(cv<T>* __restrict__ psi, uint64_t boff, BitrevArgs a) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     cv<T>* s = reinterpret_cast<cv<T>*>(smem_raw);
     constexpr int EPT = 4;  // tile elements per thread (tiles of <= 1024)
-    // The tile index enumerates the Mid bits (low) and the bits above eq;
-    // a block handles the pair (Mid, rev(Mid)) once, from the smaller Mid.
+    // block -> (outer, c, x, A); T(x) = x (x + 1) / 2 <= rem < T(x + 1)
+    const uint64_t blk = boff + blockIdx.x;
+    const uint32_t cb = a.midn & 1;
+    const uint64_t per_c = bitrev_pairs_per_outer(a) >> cb;
+    const uint64_t outer = blk / (per_c << cb);
+    uint64_t rem = blk - outer * (per_c << cb);
+    const uint64_t c = cb ? rem / per_c : 0;
+    rem -= c * per_c;
+    uint64_t x = uint64_t((sqrt(8.0 * double(rem) + 1.0) - 1.0) * 0.5);
+    while (x * (x + 1) / 2 > rem) --x;
+    while ((x + 1) * (x + 2) / 2 <= rem) ++x;
+    const uint64_t A = rem - x * (x + 1) / 2;
+    const uint64_t mid = (A << (a.hh + cb)) | (c << a.hh) | rev_bits(x, a.hh);
+    const uint64_t mid_r = rev_bits(mid, a.midn);
+    const bool two = mid_r != mid;
     const uint32_t hi_lo = a.sq + a.m - a.b;  // first Hi bit
-    const uint32_t midbits = hi_lo - a.rowlog;
-    auto active = [&](uint64_t tile) {
-        const uint64_t mid = tile & ((1ull << midbits) - 1);
-        return rev_bits(mid, a.midn) >= mid;
-    };
-    auto tile_base = [&](uint64_t tile, uint64_t md) {
-        return ((tile >> midbits) << (a.sq + a.m)) | (md << a.rowlog);
-    };
+    const uint64_t base0 = (outer << (a.sq + a.m)) | (mid << a.rowlog);
+    const uint64_t base1 = (outer << (a.sq + a.m)) | (mid_r << a.rowlog);
     // LDS rows are padded by one element so that the transposed reads of
     // consecutive lanes (stride of one row) fall in different banks.
     const uint32_t R = 1u << a.rowlog, H = 1u << a.b, E = R * H, RP = R + 1, EP = RP * H;
     const uint32_t lowmask = (1u << a.sq) - 1;
-    // Blocks stride over the tiles; the next active pair is loaded into
-    // registers while the current one is written out.
     cv<T> p0[EPT], p1[EPT];
-    auto load = [&](uint64_t tile) {
-        const uint64_t mid = tile & ((1ull << midbits) - 1), mid_r = rev_bits(mid, a.midn);
-        const uint64_t b0 = tile_base(tile, mid), b1 = tile_base(tile, mid_r);
 #pragma unroll
-        for (int k = 0; k < EPT; ++k) {
-            const uint32_t e = threadIdx.x + 256u * k;
-            if (e < E) {
-                const uint64_t off = (uint64_t(e >> a.rowlog) << hi_lo) | (e & (R - 1));
-                p0[k] = ldamp<T, kStagedNtLoads>(psi + (b0 | off));
-                if (mid_r != mid) p1[k] = ldamp<T, kStagedNtLoads>(psi + (b1 | off));
-            }
+    for (int k = 0; k < EPT; ++k) {
+        const uint32_t e = threadIdx.x + 256u * k;
+        if (e < E) {
+            const uint64_t off = (uint64_t(e >> a.rowlog) << hi_lo) | (e & (R - 1));
+            p0[k] = ldamp<T, kStagedNtLoads>(psi + (base0 | off));
+            if (two) p1[k] = ldamp<T, kStagedNtLoads>(psi + (base1 | off));
         }
-    };
-    uint64_t tile = blockIdx.x;
-    while (tile < ntiles && !active(tile)) tile += gridDim.x;
-    if (tile < ntiles) load(tile);
-    while (tile < ntiles) {
-        const uint64_t mid = tile & ((1ull << midbits) - 1), mid_r = rev_bits(mid, a.midn);
-        const bool two = mid_r != mid;
-        const uint64_t base0 = tile_base(tile, mid), base1 = tile_base(tile, mid_r);
+    }
 #pragma unroll
-        for (int k = 0; k < EPT; ++k) {
-            const uint32_t e = threadIdx.x + 256u * k;
-            if (e < E) {
-                const uint32_t r = e & (R - 1), h = e >> a.rowlog;
-                s[h * RP + r] = p0[k];
-                if (two) s[EP + h * RP + r] = p1[k];
-            }
+    for (int k = 0; k < EPT; ++k) {
+        const uint32_t e = threadIdx.x + 256u * k;
+        if (e < E) {
+            const uint32_t r = e & (R - 1), h = e >> a.rowlog;
+            s[h * RP + r] = p0[k];
+            if (two) s[EP + h * RP + r] = p1[k];
         }
-        __syncthreads();
-        uint64_t next = tile + gridDim.x;
-        while (next < ntiles && !active(next)) next += gridDim.x;
-        if (next < ntiles) load(next);
-        for (uint32_t e = threadIdx.x; e < E; e += 256) {
-            // destination (h', r') reads source (h, r) with lo' = rev(h), h' = rev(lo)
-            const uint32_t rd = e & (R - 1), hd = e >> a.rowlog;
-            const uint32_t lo_d = rd >> a.sq, out_low = rd & lowmask;
-            const uint32_t lo_s = uint32_t(rev_bits(hd, a.b)), h_s = uint32_t(rev_bits(lo_d, a.b));
-            const uint32_t src = h_s * RP + ((lo_s << a.sq) | out_low);
-            // tile0's destination data comes from tile1 (or itself) and vice versa
-            stamp<T, true>(psi + (base0 | (uint64_t(hd) << hi_lo) | rd), two ? s[EP + src] : s[src]);
-            if (two) stamp<T, true>(psi + (base1 | (uint64_t(hd) << hi_lo) | rd), s[src]);
-        }
-        __syncthreads();
-        tile = next;
+    }
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < E; e += 256) {
+        // destination (h', r') reads source (h, r) with lo' = rev(h), h' = rev(lo)
+        const uint32_t rd = e & (R - 1), hd = e >> a.rowlog;
+        const uint32_t lo_d = rd >> a.sq, out_low = rd & lowmask;
+        const uint32_t lo_s = uint32_t(rev_bits(hd, a.b)), h_s = uint32_t(rev_bits(lo_d, a.b));
+        const uint32_t src = h_s * RP + ((lo_s << a.sq) | out_low);
+        // tile0's destination data comes from tile1 (or itself) and vice versa
+        stamp<T, true>(psi + (base0 | (uint64_t(hd) << hi_lo) | rd), two ? s[EP + src] : s[src]);
+        if (two) stamp<T, true>(psi + (base1 | (uint64_t(hd) << hi_lo) | rd), s[src]);
     }
 }
 
@@ -556,16 +548,18 @@ static int launch_lds(qcs_register* r, const double2* tw, uint32_t sq, uint32_t 
                       bool inv) {
     const QftArgs a{sq, eq, sq, hi - sq + 1};
     const uint64_t nchunks = r->A >> clog;
-    const unsigned grid = unsigned(nchunks < 4096 ? nchunks : 4096);
     const size_t shm = sizeof(cv<T>) << clog;
     auto* psi = static_cast<cv<T>*>(r->psi);
     static const QftRoots roots = group_roots();
-    if (inv)
-        hipLaunchKernelGGL((k_qft_lds<T, true>), dim3(grid), dim3(256), shm, r->stream, psi, clog, nchunks, a, hi, tw,
-                           roots);
-    else
-        hipLaunchKernelGGL((k_qft_lds<T, false>), dim3(grid), dim3(256), shm, r->stream, psi, clog, nchunks, a, hi,
-                           tw, roots);
+    for (uint64_t boff = 0; boff < nchunks; boff += kMaxBlocks) {
+        const unsigned g = unsigned(std::min<uint64_t>(kMaxBlocks, nchunks - boff));
+        if (inv)
+            hipLaunchKernelGGL((k_qft_lds<T, true>), dim3(g), dim3(256), shm, r->stream, psi, clog, boff, a, hi, tw,
+                               roots);
+        else
+            hipLaunchKernelGGL((k_qft_lds<T, false>), dim3(g), dim3(256), shm, r->stream, psi, clog, boff, a, hi, tw,
+                               roots);
+    }
     return check_launch(r, "k_qft_lds");
 }
 
@@ -592,11 +586,13 @@ static int bitrev(qcs_register* r, uint32_t sq, uint32_t eq) {
     a.rowlog = sq + a.b;
     a.midlo = a.rowlog;
     a.midn = m - 2 * a.b;
-    const uint64_t ntiles = r->A >> (a.rowlog + a.b);
+    a.hh = a.midn / 2;
+    const uint64_t nblocks = (r->A >> (sq + m)) * bitrev_pairs_per_outer(a);
     const size_t EP = ((size_t(1) << a.rowlog) + 1) << a.b;  // padded tile
-    const unsigned grid = unsigned(ntiles < 8192 ? ntiles : 8192);
-    hipLaunchKernelGGL((k_bitrev_tile<T>), dim3(grid), dim3(256), 2 * EP * sizeof(cv<T>), r->stream, psi, ntiles,
-                       a);
+    for (uint64_t boff = 0; boff < nblocks; boff += kMaxBlocks) {
+        const unsigned g = unsigned(std::min<uint64_t>(kMaxBlocks, nblocks - boff));
+        hipLaunchKernelGGL((k_bitrev_tile<T>), dim3(g), dim3(256), 2 * EP * sizeof(cv<T>), r->stream, psi, boff, a);
+    }
     return check_launch(r, "k_bitrev_tile");
 }
 
