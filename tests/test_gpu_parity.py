@@ -299,20 +299,22 @@ def test_random_circuit_20_qubits_with_qft_and_samples():
     assert all(margin[i] < 1e-10 for i in bad)
 
 
+@pytest.mark.parametrize("precision,tol", [("c128", TOL64), ("c64", TOL32)])
 @pytest.mark.parametrize("sq,eq", [(0, 15), (1, 15), (2, 13), (4, 15), (3, 12), (7, 15), (0, 9), (5, 5)])
-def test_qft_fused_paths(sq, eq):
+def test_qft_fused_paths(sq, eq, precision, tol):
     """Every pass type of the fused QFT (register groups, LDS chunk, tiled /
-    pair / chunk bit reversal) against the oracle's gate sequence."""
+    pair / chunk bit reversal) against the oracle's gate sequence, in both
+    storage precisions."""
     n = 16
     rng = np.random.default_rng(sq * 31 + eq)
     psi = random_state(rng, n)
     for inverse in (False, True):
         for swap in (True, False):
-            g, o = pair(n, psi)
+            g, o = pair(n, psi, precision)
             g.qft(sq, eq, swap, inverse)
             o.qft(sq, eq, swap, inverse)
             err = np.max(np.abs(g.getRegisterStorage() - o.state()))
-            assert err < TOL64, (sq, eq, inverse, swap, err)
+            assert err < tol, (sq, eq, precision, inverse, swap, err)
 
 
 def _product_state(rng, n):
