@@ -86,3 +86,21 @@ def test_32_qubits_complex128_qft_closed_form():
         assert abs(r.getBasisStateAmplitude(k) - want) < 1e-12 / math.sqrt(N) * 64, k
     assert abs(r.norm2() - 1) < 1e-12
     r.close()
+
+
+def test_33_qubits_complex64_qft_closed_form():
+    """The QFT passes past 2^32 amplitudes in complex64; the bit reversal's
+    tile pairs (4,196,352 at 33 qubits) take two launches."""
+    n = 33
+    N = 1 << n
+    rng = np.random.default_rng(331)
+    r = QubitRegister(n, precision="c64")
+    j = (1 << 32) + int(rng.integers(1 << 32))
+    r.setToBasisState(j)
+    r.qft()
+    scale = 1 / math.sqrt(N)
+    for k in [int(x) for x in rng.integers(N, size=32)] + [0, N - 1, 1 << 32, (1 << 32) - 1]:
+        want = np.exp(2j * np.pi * ((j * k) % N) / N) * scale
+        assert abs(r.getBasisStateAmplitude(k) - want) < 2e-5 * scale, k
+    assert abs(r.norm2() - 1) < 1e-5
+    r.close()
