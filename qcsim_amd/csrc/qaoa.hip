@@ -220,6 +220,30 @@ __device__ void rx_round(float2* t, uint32_t tid, uint32_t tb, uint32_t lo, floa
     __syncthreads();
 }
 
+// Tiles of 4096 amplitudes: wave w owns the 1024 elements with bits 10..11
+// = w for the Rx rounds on qubits 4..7 and 0..3 and for the store, so those
+// steps exchange data only within the wave and need no block barrier. In a
+// round on qubits lo..lo+3 (lo = 0 or 4) lane bits 0..3 give the other four
+// of bits 0..7 and lane bits 4..5 give bits 8..9.
+__device__ __forceinline__ uint32_t wave_orbit_base(uint32_t tid, uint32_t lo) {
+    const uint32_t lane = tid & 63, w = tid >> 6;
+    return ((lane & 15) << (lo == 0 ? 4 : 0)) | ((lane >> 4) << 8) | (w << 10);
+}
+
+__device__ __forceinline__ void rx_wave(float2* t, uint32_t tid, uint32_t lo, float c, float s) {
+    const uint32_t lb = wave_orbit_base(tid, lo);
+    float2 y[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) y[k] = t[swz(lb | (uint32_t(k) << lo))];
+    rx_reg<4>(y, c, s);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) t[swz(lb | (uint32_t(k) << lo))] = y[k];
+}
+
+// Orders this wave's LDS writes before its later LDS reads (the data a lane
+// reads was written by another lane of the same wave).
+__device__ __forceinline__ void lds_wave_sync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
+
 // Terms of the cost C(s) = sum J z z + sum h z / 2 for a tile of 2^tb
 // amplitudes whose bits tb.. are fixed (by `base`): K = the part of C on the
 // fixed spins alone; F[i] (i < tb) = h_i / 2 + sum of J_ij z_j over the fixed
@@ -375,8 +399,17 @@ __device__ void tile_layer(float2* t, uint32_t tid, uint64_t base, double gamma,
 #pragma unroll
     for (uint32_t m = 0; m < (1u << M); ++m) t[swz(tid + (m << 8))] = y[m];
     __syncthreads();
-    rx_round<4>(t, tid, tb, 4, c, s);
-    rx_round<4>(t, tid, tb, 0, c, s);
+    if constexpr (M == 4) {
+        // wave-local rounds; the caller orders the result before other waves
+        // read it
+        rx_wave(t, tid, 4, c, s);
+        lds_wave_sync();
+        rx_wave(t, tid, 0, c, s);
+        lds_wave_sync();
+    } else {
+        rx_round<4>(t, tid, tb, 4, c, s);
+        rx_round<4>(t, tid, tb, 0, c, s);
+    }
 }
 
 // A tile unit: one layer on tiles tile0 .. tile0+ntile-1 of an instance,
@@ -393,10 +426,19 @@ __device__ void tile_run(float2* t, float2* psi, uint64_t tile0, uint32_t ntile,
         asm volatile("" : "+v"(tid));
         __builtin_assume(tid < 256u);
         tile_layer<M>(t, tid, base, gamma, c, s, I, P, TC, W, first ? IN_PLUS : IN_GLOBAL, psi + base, amp);
+        if constexpr (M == 4) {
+            // each wave stores the 1024 elements it rotated last
 #pragma unroll
-        for (uint32_t r = 0; r < C / 256; ++r) {
-            const uint32_t i = tid + 256 * r;
-            stamp<float, true>(psi + base + i, t[swz(i)]);
+            for (uint32_t r = 0; r < 16; ++r) {
+                const uint32_t i = ((tid >> 6) << 10) | (r << 6) | (tid & 63);
+                stamp<float, true>(psi + base + i, t[swz(i)]);
+            }
+        } else {
+#pragma unroll
+            for (uint32_t r = 0; r < C / 256; ++r) {
+                const uint32_t i = tid + 256 * r;
+                stamp<float, true>(psi + base + i, t[swz(i)]);
+            }
         }
     }
 }
@@ -416,6 +458,7 @@ __device__ void small_run(float2* t, const double* an, uint32_t p, const QInst& 
         __builtin_assume(tid < 256u);
         tile_layer<M>(t, tid, 0, an[2 * l], float(cb), float(sb), I, P, TC, W, l ? IN_LDS : IN_PLUS, nullptr, amp);
     }
+    __syncthreads();  // the energy pass reads every wave's elements
 }
 
 __device__ __forceinline__ void tile_run_tb(uint32_t tb, float2* t, float2* psi, uint64_t tile0, uint32_t ntile, double gamma,
