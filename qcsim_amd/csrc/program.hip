@@ -38,6 +38,17 @@ static qcs_gate adjoint_gate(const qcs_gate& g) {
     if (isd) a.traits |= QCS_TRAIT_ISWAP;
     return a;
 }
+// Upper bound of the parameter bytes one block's launch stages (each
+// staging is rounded up to 256 B): a dense block takes at most one staging
+// slot; a generic 2- or 3-qubit gate may run k_nq_split, which stages its
+// matrix as D*D real parts then D*D imaginary parts; every other launch
+// passes its parameters as kernel arguments.
+static size_t param_bound(const qcs_register* r, const qcs_gate& g) {
+    if ((g.flags & QCS_GATE_DENSE) || g.k > 3) return (r->mat_bytes + 255) & ~size_t(255);
+    if (g.k < 2) return 0;
+    const size_t D = size_t(1) << g.k;
+    return (2 * D * D * sizeof(double) + 255) & ~size_t(255);
+}
 }  // namespace qcs
 
 using namespace qcs;
@@ -45,7 +56,7 @@ using namespace qcs;
 struct qcs_program {
     qcs_register* r = nullptr;
     std::vector<qcs_gate> blocks;  // the (fused) sequence, kept for re-capture
-    size_t ndense = 0;
+    size_t param_bytes = 0;        // upper bound of what the launches stage
     void* psi = nullptr;  // state buffer the graph was captured for
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -62,7 +73,7 @@ static int capture(qcs_program* p) {
     p->exec = nullptr;
     p->graph = nullptr;
     ParamSink sink;
-    std::vector<char> host(std::max<size_t>(p->ndense, 1) * r->mat_bytes);
+    std::vector<char> host(std::max<size_t>(p->param_bytes, 256));
     sink.host = host.data();
     sink.cap = host.size();
     if (!p->params && hipMalloc(&p->params, sink.cap) != hipSuccess) {
@@ -134,7 +145,7 @@ int qcs_program_create(qcs_register* r, const qcs_gate* gates, uint32_t n, uint3
     } else {
         p->blocks.swap(seq);
     }
-    for (const auto& g : p->blocks) p->ndense += (g.flags & QCS_GATE_DENSE) || g.k > 3;
+    for (const auto& g : p->blocks) p->param_bytes += param_bound(r, g);
     if (const int e = capture(p)) {
         qcs_program_destroy(p);
         return e;

@@ -172,3 +172,102 @@ def test_circuit_file_runs_on_the_engine(tmp_path):
     prog.close()
     for r in (a, b, c):
         r.close()
+
+
+def oracle_run(n, psi, seq, adjoint=False):
+    """The oracle's gate-by-gate run of seq, or with adjoint=True of its
+    inverse: the gates reversed with conjugate-transposed matrices and no
+    traits, which is what the reference's Uncompute applies
+    (QubitRegister.h:571-590)."""
+    from oracle.oracle import OracleRegister
+
+    ref = OracleRegister(n)
+    ref.set_state(psi)
+    if not adjoint:
+        for gate, qs in seq:
+            ref.apply_gate(gate, *qs)
+    else:
+        for gate, qs in reversed(seq):
+            qq = list(qs) + [0] * (3 - len(qs))
+            ref.apply_raw(gate.getQubitsNumber(), 0, qq[0], qq[1], qq[2], gate.getRawOperatorMatrix().conj().T)
+    out = ref.state()
+    ref.close()
+    return out
+
+
+@pytest.mark.parametrize("adjoint", [False, True])
+@pytest.mark.parametrize("width", [0, 4, 5])
+def test_program_matches_oracle(width, adjoint):
+    """Forward (Compute) and adjoint (Uncompute) programs, unfused and fused,
+    against the oracle's gate-by-gate run at 1e-12."""
+    n = 14
+    rng = np.random.default_rng(50 + width + 7 * adjoint)
+    seq = random_sequence(rng, n, 160)
+    psi = random_state(rng, n)
+    r = fresh(n, psi)
+    prog = r.compile_program(seq, fuse_width=width, adjoint=adjoint)
+    prog.run()
+    err = float(np.max(np.abs(r.getRegisterStorage() - oracle_run(n, psi, seq, adjoint))))
+    print(f"program width {width} adjoint {adjoint}: max-abs vs oracle {err:.2e}")
+    assert err < 1e-12
+    prog.close()
+    r.close()
+
+
+def qft_applied_gates(n):
+    """The reference's QFT sequence on qubits 0..n-1 with swaps
+    (QuantumFourierTransform.h:35-60, QubitsSwapper.h:23-33), as traitless
+    recorded gates, which is how Compute/Uncompute replays it."""
+    import math
+
+    seq = []
+    h = G.HadamardGate().getRawOperatorMatrix()
+    seq.append((G.AppliedGate(h), [n - 1]))
+    for cur in range(n - 1, 0, -1):
+        phase = math.pi / 2
+        for ctrl in range(cur - 1, -1, -1):
+            seq.append((G.AppliedGate(G.ControlledPhaseShiftGate(phase).getRawOperatorMatrix()), [cur, ctrl]))
+            phase *= 0.5
+        seq.append((G.AppliedGate(h), [cur - 1]))
+    sw = G.SwapGate().getRawOperatorMatrix()
+    a, b = 0, n - 1
+    while a < b:
+        seq.append((G.AppliedGate(sw), [a, b]))
+        a += 1
+        b -= 1
+    return seq
+
+
+@pytest.mark.parametrize("adjoint", [False, True])
+def test_program_with_many_generic_gates(adjoint):
+    """A 17-qubit QFT recorded as traitless gates (136 controlled phases and 8
+    swaps run as generic 4x4 blocks, each staging its matrix) compiles and
+    replays like the oracle; so does a run of 220 generic 2- and 3-qubit
+    gates."""
+    n = 17
+    rng = np.random.default_rng(61)
+    psi = random_state(rng, n)
+    seq = qft_applied_gates(n)
+    assert sum(g.getQubitsNumber() == 2 for g, _ in seq) == 144
+    r = fresh(n, psi)
+    prog = r.compile_program(seq, adjoint=adjoint)
+    prog.run()
+    err = float(np.max(np.abs(r.getRegisterStorage() - oracle_run(n, psi, seq, adjoint))))
+    print(f"17-qubit recorded QFT adjoint {adjoint}: max-abs vs oracle {err:.2e}")
+    assert err < 1e-12
+    prog.close()
+    from refmath import random_unitary
+
+    seq = []
+    for i in range(220):
+        k = 2 if i % 4 else 3
+        qs = [int(x) for x in rng.choice(n, size=k, replace=False)]
+        seq.append((G.AppliedGate(random_unitary(rng, 1 << k)), qs))
+    r.setRegisterStorageFastNoNormalize(psi.copy())
+    prog = r.compile_program(seq, adjoint=adjoint)
+    prog.run()
+    err = float(np.max(np.abs(r.getRegisterStorage() - oracle_run(n, psi, seq, adjoint))))
+    print(f"220 generic gates adjoint {adjoint}: max-abs vs oracle {err:.2e}")
+    assert err < 1e-12
+    prog.close()
+    r.close()
