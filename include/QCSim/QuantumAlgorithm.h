@@ -56,6 +56,33 @@ public:
     }
     double GetQubitProbability(size_t qubit) const { return reg.GetQubitProbability(qubit); }
 
+    // |<reg|state>|^2 against a known state (QuantumAlgorithm.h:143-146)
+    double stateFidelity(const VectorClass& state) const { return reg.stateFidelity(state); }
+
+    // Host-side twin of the register's AdjustPhaseAndNormalize for a vector
+    // the caller holds (QuantumAlgorithm.h:148-162): every entry is divided
+    // by entry 0 (by 1 when |entry 0| <= 1e-5), then the vector is scaled
+    // to unit norm.
+    static void AdjustPhaseAndNormalize(VectorClass& regVals) {
+        const size_t count = static_cast<size_t>(regVals.size());
+        if (count == 0) return;
+        const std::complex<double> first = regVals[0];
+        const std::complex<double> div = std::abs(first) > 1E-5 ? first : std::complex<double>(1, 0);
+        double sum = 0;
+        for (size_t i = 0; i < count; ++i) {
+            regVals[i] /= div;
+            sum += std::norm(regVals[i]);
+        }
+        const double scale = 1. / std::sqrt(sum);
+        for (size_t i = 0; i < count; ++i) regVals[i] *= scale;
+    }
+
+    // text dumps of the register (QuantumAlgorithm.h:164-172)
+    bool writeToFile(const std::string& name, bool amplitude = true, bool append = false) const {
+        return reg.writeToFile(name, amplitude, append);
+    }
+    void displayRegister() const { reg.displayRegister(); }
+
     void ComputeStart() { reg.ComputeStart(); }
     void ComputeEnd() { reg.ComputeEnd(); }
     void ComputeClear() { reg.ComputeClear(); }

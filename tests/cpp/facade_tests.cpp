@@ -251,8 +251,21 @@ static bool stateManagement() {
     if (!approx(c->getBasisStateAmplitude(4), 1. / std::sqrt(2.))) return false;
     Eigen::VectorXcd v = Eigen::VectorXcd::Zero(16);
     v(3) = 1;
-    reg.setRegisterStorageFastNoNormalize(v);  // v receives the old state
-    if (!approx(reg.getBasisStateAmplitude(3), 1.) || !approx(v(4), 1. / std::sqrt(2.))) return false;
+    (void)reg.getRegisterStorage();            // the host mirror holds the state now
+    reg.setRegisterStorageFastNoNormalize(v);  // so v receives the old state
+    if (!approx(reg.getBasisStateAmplitude(3), 1.) || v.size() != 16 || !approx(v(4), 1. / std::sqrt(2.)))
+        return false;
+    if (!approx(reg.getRegisterStorage()(3), 1.)) return false;  // mirror = the uploaded state
+    {
+        // after a mutation the mirror is stale: the vector comes back empty
+        // instead of costing a download of the whole state
+        QC::QubitRegister<> t(4);
+        t.ApplyGate(h, 1);
+        Eigen::VectorXcd u = Eigen::VectorXcd::Zero(16);
+        u(5) = 1;
+        t.setRegisterStorageFastNoNormalize(u);
+        if (u.size() != 0 || !approx(t.getBasisStateAmplitude(5), 1.)) return false;
+    }
     Eigen::VectorXcd w = Eigen::VectorXcd::Zero(16);
     w(7) = 2;
     QC::QubitRegister<> r2(4, w);  // the (N, v) ctor takes v's contents
@@ -282,6 +295,71 @@ static bool samplingAndOperators() {
     p(0, 1) = p(1, 0) = p(2, 2) = p(3, 3) = 1;
     reg.ApplyOperatorMatrix(p);
     return approx(reg.getBasisStateAmplitude(0), cd(0, 1. / std::sqrt(2.)));
+}
+
+// One-qubit matrix on qubit q of a host vector (test helper).
+static void hostApply1(Eigen::VectorXcd& v, const Eigen::MatrixXcd& m, size_t q) {
+    const size_t bit = size_t(1) << q;
+    for (size_t s = 0; s < size_t(v.size()); ++s) {
+        if (s & bit) continue;
+        const cd a = v(s), b = v(s | bit);
+        v(s) = m(0, 0) * a + m(0, 1) * b;
+        v(s | bit) = m(1, 0) * a + m(1, 1) * b;
+    }
+}
+
+// ApplyOperatorMatrix with a dense operator (QubitRegister.h:499-505) on the
+// device, and ExpectationValue's Pauli-product path (:646-660) against
+// host arithmetic and against the copy path.
+static bool operatorsAndPauli() {
+    const size_t n = 6, N = size_t(1) << n;
+    QC::QubitRegister<> reg(n);
+    QC::Gates::RyGate<> ry;
+    QC::Gates::CNOTGate<> cx;
+    for (size_t q = 0; q < n; ++q) {
+        ry.SetTheta(0.3 + 0.4 * double(q));
+        reg.ApplyGate(ry, q);
+    }
+    for (size_t q = 0; q + 1 < n; ++q) reg.ApplyGate(cx, q + 1, q);
+    const Eigen::VectorXcd v = reg.getRegisterStorage();
+    Eigen::MatrixXcd m = Eigen::MatrixXcd::Zero(N, N);
+    for (size_t i = 0; i < N; ++i)
+        for (size_t j = 0; j < N; ++j) m(i, j) = cd(std::sin(0.37 * double(i * N + j)), std::cos(0.11 * double(i + 3 * j)));
+    Eigen::VectorXcd want = Eigen::VectorXcd::Zero(N);
+    for (size_t i = 0; i < N; ++i)
+        for (size_t j = 0; j < N; ++j) want(i) += m(i, j) * v(j);
+    reg.ApplyOperatorMatrix(m);
+    const Eigen::VectorXcd got = reg.getRegisterStorage();
+    for (size_t i = 0; i < N; ++i)
+        if (!approx(got(i), want(i), 1e-11)) return false;
+    reg.setRegisterStorage(v);
+
+    QC::Gates::PauliXGate<> X;
+    QC::Gates::PauliYGate<> Y;
+    QC::Gates::PauliZGate<> Z;
+    std::vector<QC::Gates::AppliedGate<>> gates{
+        QC::Gates::AppliedGate<>(X.getRawOperatorMatrix(), 0), QC::Gates::AppliedGate<>(Z.getRawOperatorMatrix(), 2),
+        QC::Gates::AppliedGate<>(Y.getRawOperatorMatrix(), 3), QC::Gates::AppliedGate<>(Z.getRawOperatorMatrix(), 0)};
+    Eigen::VectorXcd w = v;
+    for (const auto& g : gates) hostApply1(w, g.getRawOperatorMatrix(), g.getQubit1());
+    cd ev = 0;
+    for (size_t s = 0; s < N; ++s) ev += std::conj(v(s)) * w(s);
+    const cd fast = reg.ExpectationValue(gates);
+    if (!approx(fast, ev, 1e-12)) return false;
+    // Z X on qubit 0 = i Y: the masks carry Y on qubits 0 and 3, Z on 2
+    const double p = reg.PauliExpectation(0b1001, 0b1101);
+    if (!approx(fast, cd(0, 1) * p, 1e-12)) return false;
+    // a Hadamard in the list takes the copy path; the state is restored
+    QC::Gates::HadamardGate<> H;
+    gates.emplace_back(H.getRawOperatorMatrix(), 5);
+    hostApply1(w, H.getRawOperatorMatrix(), 5);
+    ev = 0;
+    for (size_t s = 0; s < N; ++s) ev += std::conj(v(s)) * w(s);
+    if (!approx(reg.ExpectationValue(gates), ev, 1e-12)) return false;
+    const Eigen::VectorXcd after = reg.getRegisterStorage();
+    for (size_t s = 0; s < N; ++s)
+        if (!approx(after(s), v(s), 1e-14)) return false;
+    return true;
 }
 
 // Compute / Uncompute (QubitRegister.h:536-590) repeated, with the recording
@@ -357,6 +435,140 @@ static bool maxCutQaoa() {
     return minStates.count(state) == 1 && maxcut.GetExactEvaluations() % 9 == 0;
 }
 
+// A QuantumAlgorithm subclass used the way SimulationTests.cpp:66-79 uses
+// one: an exact vector from a host operator, the static
+// AdjustPhaseAndNormalize on it, the member one on the register, then
+// stateFidelity; plus writeToFile / displayRegister (QuantumAlgorithm.h:143-172).
+class RotationAlgorithm : public QC::QuantumAlgorithm<> {
+public:
+    explicit RotationAlgorithm(size_t n) : QC::QuantumAlgorithm<>(n) {}
+    size_t Execute() override {
+        QC::Gates::RyGate<> ry;
+        QC::Gates::RzGate<> rz;
+        QC::Gates::CNOTGate<> cx;
+        for (size_t q = 0; q < getNrQubits(); ++q) {
+            ry.SetTheta(0.4 + 0.3 * double(q));
+            ApplyGate(ry, q);
+            rz.SetTheta(0.7 - 0.2 * double(q));
+            ApplyGate(rz, q);
+        }
+        for (size_t q = 0; q + 1 < getNrQubits(); ++q) ApplyGate(cx, q + 1, q);
+        return 0;
+    }
+};
+
+static bool algorithmInterface() {
+    const size_t n = 3, N = size_t(1) << n;
+    RotationAlgorithm sim(n);
+    // exact result: the same product built as a dense operator on the host
+    auto kron1 = [&](const Eigen::MatrixXcd& g, size_t q) {
+        Eigen::MatrixXcd m = Eigen::MatrixXcd::Zero(N, N);
+        for (size_t r = 0; r < N; ++r)
+            for (size_t c = 0; c < N; ++c) {
+                if ((r ^ c) & ~(size_t(1) << q)) continue;
+                m(r, c) = g((r >> q) & 1, (c >> q) & 1);
+            }
+        return m;
+    };
+    Eigen::VectorXcd ex = Eigen::VectorXcd::Zero(N);
+    ex(0) = 1;
+    QC::Gates::RyGate<> ry;
+    QC::Gates::RzGate<> rz;
+    for (size_t q = 0; q < n; ++q) {
+        ry.SetTheta(0.4 + 0.3 * double(q));
+        ex = kron1(ry.getRawOperatorMatrix(), q) * ex;
+        rz.SetTheta(0.7 - 0.2 * double(q));
+        ex = kron1(rz.getRawOperatorMatrix(), q) * ex;
+    }
+    for (size_t q = 0; q + 1 < n; ++q) {  // CNOT: target q + 1, control q
+        Eigen::VectorXcd t = ex;
+        for (size_t s = 0; s < N; ++s)
+            if (s >> q & 1) t(s) = ex(s ^ (size_t(1) << (q + 1)));
+        ex = t;
+    }
+    // a global phase and scale the normalisation must remove
+    for (size_t s = 0; s < N; ++s) ex(s) *= cd(0.0, 3.0);
+    sim.AdjustPhaseAndNormalize(ex);  // static, called through the object
+    if (std::abs(ex(0).imag()) > 1e-15 || ex(0).real() <= 0) return false;
+    double nrm = 0;
+    for (size_t s = 0; s < N; ++s) nrm += std::norm(ex(s));
+    if (std::abs(nrm - 1) > 1e-14) return false;
+
+    sim.setToBasisState(0);
+    sim.Execute();
+    sim.AdjustPhaseAndNormalize();
+    const double fidelity = sim.stateFidelity(ex);
+    if (std::abs(fidelity - 1) > 1e-12) return false;
+    const Eigen::VectorXcd got = sim.getRegisterStorage();
+    for (size_t s = 0; s < N; ++s)
+        if (!approx(got(s), ex(s), 1e-12)) return false;
+    // fidelity against an orthogonal state
+    Eigen::VectorXcd orth = Eigen::VectorXcd::Zero(N);
+    orth(0) = -std::conj(ex(1));
+    orth(1) = std::conj(ex(0));
+    if (sim.stateFidelity(orth) > 1e-24) return false;
+
+    // text dump: one "index re im" (or "index probability") line per state
+    const std::string path = "/tmp/qcs_facade_dump.txt";
+    if (!sim.writeToFile(path)) return false;
+    if (!sim.writeToFile(path, false, true)) return false;  // appended
+    std::FILE* f = std::fopen(path.c_str(), "r");
+    if (!f) return false;
+    size_t idx = 0, lines = 0;
+    double re = 0, im = 0;
+    bool ok = true;
+    for (; lines < N && std::fscanf(f, "%zu %lf %lf", &idx, &re, &im) == 3; ++lines)
+        ok = ok && idx == lines && approx(cd(re, im), got(lines), 1e-15);
+    for (size_t k = 0; k < N && std::fscanf(f, "%zu %lf", &idx, &re) == 2; ++k, ++lines)
+        ok = ok && idx == k && std::abs(re - std::norm(got(k))) < 1e-15;
+    std::fclose(f);
+    std::remove(path.c_str());
+    if (!ok || lines != 2 * N) return false;
+    sim.displayRegister();
+    return true;
+}
+
+// Compute / Uncompute of a 17-qubit QFT recorded gate by gate: 136
+// controlled phases and 8 swaps replay as generic 4x4 blocks, each staging
+// its matrix in the compiled program's parameter memory.
+static bool recordedQftUncompute() {
+    const size_t n = 17;
+    QC::QubitRegister<> reg(n);
+    QC::Gates::HadamardGate<> h;
+    QC::Gates::RyGate<> ry;
+    for (size_t q = 0; q < n; ++q) {
+        ry.SetTheta(0.1 + 0.05 * double(q));
+        reg.ApplyGate(ry, q);
+    }
+    const Eigen::VectorXcd in = reg.getRegisterStorage();
+    QC::SubAlgo::QuantumFourierTransform<> qft(n);
+    reg.ComputeStart();
+    qft.QFT(reg);
+    reg.ComputeEnd();
+    const Eigen::VectorXcd once = reg.getRegisterStorage();
+    for (int rep = 0; rep < 2; ++rep) {
+        reg.Uncompute();
+        const Eigen::VectorXcd back = reg.getRegisterStorage();
+        for (size_t s = 0; s < (size_t(1) << n); ++s)
+            if (!approx(back(s), in(s), 1e-12)) return false;
+        reg.Compute();
+        const Eigen::VectorXcd again = reg.getRegisterStorage();
+        for (size_t s = 0; s < (size_t(1) << n); ++s)
+            if (!approx(again(s), once(s), 1e-12)) return false;
+    }
+    // the recorded sequence is the QFT: compare with the fused passes
+    QC::QubitRegister<> fused(n);
+    for (size_t q = 0; q < n; ++q) {
+        ry.SetTheta(0.1 + 0.05 * double(q));
+        fused.ApplyGate(ry, q);
+    }
+    qft.QFT(fused);
+    const Eigen::VectorXcd f = fused.getRegisterStorage();
+    for (size_t s = 0; s < (size_t(1) << n); ++s)
+        if (!approx(f(s), once(s), 1e-12)) return false;
+    return true;
+}
+
 int main() {
     const std::vector<std::pair<const char*, bool (*)()>> tests = {
         {"measurementCollapse", measurementCollapse},
@@ -372,6 +584,9 @@ int main() {
         {"computeUncomputeReplay", computeUncomputeReplay},
         {"isingModelQaoa", isingModelQaoa},
         {"maxCutQaoa", maxCutQaoa},
+        {"algorithmInterface", algorithmInterface},
+        {"operatorsAndPauli", operatorsAndPauli},
+        {"recordedQftUncompute", recordedQftUncompute},
     };
     int failed = 0;
     for (const auto& t : tests) {
