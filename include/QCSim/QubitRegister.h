@@ -17,6 +17,7 @@
 // the draws the reference consumes, with prob = 1 - u.
 #pragma once
 
+#include <array>
 #include <chrono>
 #include <climits>
 #include <cstdint>
@@ -55,6 +56,59 @@ template <class GateT> uint32_t gate_traits(const GateT& g) {
     if (g.IsISwapGate()) t |= QCS_TRAIT_ISWAP;
     if (g.IsISwapDagGate()) t |= QCS_TRAIT_ISWAPDAG;
     return t;
+}
+
+// A list of single-qubit gates whose product is coef * P(x, z) with P a
+// Pauli string (qubit q: X for bit q of x only, Z for z only, Y = iXZ for
+// both): per qubit the 2x2 product (later gates on the left) must be a
+// scalar multiple of I, X, Y or Z, exactly.
+template <class AppliedGateT>
+bool gates_as_pauli(const std::vector<AppliedGateT>& gates, size_t nqubits, std::complex<double>& coef,
+                    uint64_t& x, uint64_t& z) {
+    using cd = std::complex<double>;
+    std::map<size_t, std::array<cd, 4>> per;
+    for (const auto& g : gates) {
+        if (g.getQubitsNumber() != 1) return false;
+        const size_t q = g.getQubit1();
+        if (q >= nqubits || q >= 64) return false;
+        const auto& m = g.getRawOperatorMatrix();
+        const std::array<cd, 4> a{cd(m(0, 0)), cd(m(0, 1)), cd(m(1, 0)), cd(m(1, 1))};
+        auto it = per.find(q);
+        if (it == per.end()) {
+            per.emplace(q, a);
+        } else {
+            const std::array<cd, 4> b = it->second;
+            it->second = {a[0] * b[0] + a[1] * b[2], a[0] * b[1] + a[1] * b[3], a[2] * b[0] + a[3] * b[2],
+                          a[2] * b[1] + a[3] * b[3]};
+        }
+    }
+    coef = 1;
+    x = z = 0;
+    for (const auto& kv : per) {
+        const std::array<cd, 4>& m = kv.second;
+        const uint64_t bit = uint64_t(1) << kv.first;
+        if (m[1] == cd(0) && m[2] == cd(0)) {
+            if (m[0] == m[3]) coef *= m[0];
+            else if (m[0] == -m[3]) {
+                coef *= m[0];
+                z |= bit;
+            } else
+                return false;
+        } else if (m[0] == cd(0) && m[3] == cd(0)) {
+            if (m[1] == m[2]) {
+                coef *= m[1];
+                x |= bit;
+            } else if (m[2] == -m[1]) {  // c Y has entry (1, 0) = i c
+                coef *= cd(0, -1) * m[2];
+                x |= bit;
+                z |= bit;
+            } else
+                return false;
+        } else {
+            return false;
+        }
+    }
+    return true;
 }
 
 // Row-major interleaved (re, im) copy of a matrix.
@@ -110,6 +164,7 @@ public:
     virtual ~QubitRegister() {
         progFwd_.reset();  // programs first: they sync the register's stream
         progAdj_.reset();
+        if (shadow_) qcs_destroy(shadow_);
         if (saved_) qcs_destroy(saved_);
         if (h_) qcs_destroy(h_);
     }
@@ -236,8 +291,9 @@ public:
         for (const auto& gate : gates) ApplyGate(gate);
     }
 
-    // Dense 2^n x 2^n operator. Diagonal operators run on the GPU in one pass;
-    // a general matrix is applied on the host (it is O(4^n) either way).
+    // Dense 2^n x 2^n operator on the GPU: a diagonal operator in one pass
+    // over the state, any other as an FP64 matrix-vector product with the
+    // matrix streamed in panels of rows (registers of up to 21 qubits).
     void ApplyOperatorMatrix(const MatrixClass& m) {
         bool diag = true;
         for (size_t i = 0; i < NrBasisStates && diag; ++i)
@@ -253,12 +309,11 @@ public:
                 d[2 * i + 1] = m(i, i).imag();
             }
             detail::qcs_check(qcs_apply_diagonal(h_, d.data(), NrBasisStates));
-            dirty_ = true;
         } else {
-            VectorClass v = getRegisterStorage();
-            VectorClass r = m * v;
-            upload(r);
+            const std::vector<double> rows = detail::flatten(m);
+            detail::qcs_check(qcs_apply_matrix(h_, rows.data(), NrBasisStates));
         }
+        dirty_ = true;
         if (recordGates) {
             computeGates.emplace_back(Gates::AppliedGate<MatrixClass>(m));
             ++computeGen_;
@@ -280,21 +335,28 @@ public:
         Normalize();
     }
 
-    // warning: vals must be normalised and have the right size (no check)
+    // warning: vals must be normalised and have the right size (no check).
+    // The reference swaps vals with its storage (:521-524). Here vals is
+    // uploaded; it receives the previous contents when the host mirror holds
+    // them already (no mutation since the last getRegisterStorage), and is
+    // left empty otherwise, so no state-sized download is ever made.
     void setRegisterStorageFastNoNormalize(VectorClass& vals) {
-        VectorClass old = getRegisterStorage();
+        const bool mirrored = !dirty_;
         upload(vals);
-        vals.swap(old);
+        if (mirrored) {
+            mirror_.swap(vals);  // mirror = the uploaded state, vals = the old one
+            dirty_ = false;
+        } else {
+            vals = VectorClass();
+        }
     }
 
+    // |<reg|state>|^2 (:527-534): the vector streams through the device in
+    // chunks, no second register is made.
     double stateFidelity(const VectorClass& state) const {
         if (size_t(state.size()) != NrBasisStates) return 0;
-        qcs_register* other = nullptr;
-        detail::qcs_check(qcs_create(uint32_t(NrQubits), QCS_COMPLEX128, &other));
-        std::unique_ptr<qcs_register, int (*)(qcs_register*)> guard(other, qcs_destroy);
-        detail::qcs_check(qcs_set_state(other, reinterpret_cast<const double*>(state.data()), NrBasisStates));
         double z[2];
-        detail::qcs_check(qcs_inner(h_, other, z));
+        detail::qcs_check(qcs_inner_host(h_, reinterpret_cast<const double*>(state.data()), NrBasisStates, z));
         return std::norm(std::complex<double>(z[0], z[1]));
     }
 
@@ -371,18 +433,44 @@ public:
         return size_t(out);
     }
 
+    // <psi|G|psi> for the product G of the gates (:646-660). A list of
+    // single-qubit gates that are multiples of Pauli matrices (the special
+    // case the reference notes at :649) takes one read-only FP64 pass; any
+    // other list runs on a copy of the state held in a scratch register that
+    // is made on first use and kept.
     std::complex<double> ExpectationValue(const std::vector<Gates::AppliedGate<MatrixClass>>& gates) {
         if (gates.empty()) return 1.;
-        qcs_register* saved = nullptr;
-        detail::qcs_check(qcs_create(uint32_t(NrQubits), QCS_COMPLEX128, &saved));
-        std::unique_ptr<qcs_register, int (*)(qcs_register*)> guard(saved, qcs_destroy);
-        detail::qcs_check(qcs_copy_state(saved, h_));
+        std::complex<double> coef;
+        uint64_t x = 0, z = 0;
+        if (detail::gates_as_pauli(gates, NrQubits, coef, x, z)) {
+            double v = 0;
+            detail::qcs_check(qcs_pauli_expectation(h_, 1, &x, &z, &v));
+            return coef * v;
+        }
+        if (!shadow_) detail::qcs_check(qcs_create(uint32_t(NrQubits), QCS_COMPLEX128, &shadow_));
+        detail::qcs_check(qcs_copy_state(shadow_, h_));
         ApplyGates(gates);
-        double z[2];
-        detail::qcs_check(qcs_inner(saved, h_, z));
-        detail::qcs_check(qcs_swap_state(h_, saved));  // restore
+        double zz[2];
+        detail::qcs_check(qcs_inner(shadow_, h_, zz));
+        detail::qcs_check(qcs_swap_state(h_, shadow_));  // restore
         dirty_ = true;
-        return {z[0], z[1]};
+        return {zz[0], zz[1]};
+    }
+
+    // ---- engine extensions: Pauli strings ----------------------------------
+    // <psi|P|psi> for P = i^{|x & z|} X^x Z^z (qubit q: X for bit q of x only,
+    // Z for z only, Y for both), FP64; many strings in one call share passes.
+    std::vector<double> PauliExpectation(const std::vector<uint64_t>& xmask, const std::vector<uint64_t>& zmask) const {
+        if (xmask.size() != zmask.size()) throw std::invalid_argument("mask vectors differ in length");
+        std::vector<double> out(xmask.size());
+        detail::qcs_check(
+            qcs_pauli_expectation(h_, uint32_t(xmask.size()), xmask.data(), zmask.data(), out.data()));
+        return out;
+    }
+    double PauliExpectation(uint64_t xmask, uint64_t zmask) const {
+        double v = 0;
+        detail::qcs_check(qcs_pauli_expectation(h_, 1, &xmask, &zmask, &v));
+        return v;
     }
 
     std::unique_ptr<QubitRegister<VectorClass, MatrixClass>> Clone() const {
@@ -545,6 +633,7 @@ private:
 
     qcs_register* h_ = nullptr;
     qcs_register* saved_ = nullptr;
+    qcs_register* shadow_ = nullptr;  // ExpectationValue's copy of the state
     mutable VectorClass mirror_;
     mutable bool dirty_ = true;
     uint64_t computeGen_ = 0;  // bumped whenever the recording changes

@@ -139,8 +139,20 @@ int qcs_program_info(const qcs_program* p, uint32_t* launches);
 int qcs_program_destroy(qcs_program* p);
 
 /* Diagonal operator on the whole register: psi[s] *= diag[s]
- * (ApplyOperatorMatrix with a diagonal matrix, QubitRegister.h:499-505). */
+ * (ApplyOperatorMatrix with a diagonal matrix, QubitRegister.h:499-505).
+ * The host diagonal streams through the register's transfer buffer (at most
+ * 64 MiB, allocated on first use and kept) in chunks. */
 int qcs_apply_diagonal(qcs_register* r, const double* diag, uint64_t n);
+
+/* Dense operator on the whole register: psi <- M psi, M a row-major
+ * dim x dim complex matrix (interleaved re, im) in host memory, dim = 2^n
+ * (ApplyOperatorMatrix, QubitRegister.h:499-505; the reference's callers
+ * are the oracle-based algorithms, GroverAlgorithm.h:97,111,
+ * BernsteinVazirani.h:85, DeutschJozsa.h:126, NQubitsQuantumGate.h:22).
+ * M streams through the transfer buffer in panels of whole rows; each row
+ * is one FP64 dot product on the device. Registers of up to 21 qubits
+ * (complex128: 2 * 16 * 2^n bytes must fit the 64 MiB buffer). */
+int qcs_apply_matrix(qcs_register* r, const double* mat, uint64_t dim);
 
 /* ---- QFT (QuantumFourierTransform.h:35-87, QubitsSwapper.h:23-33) ----- */
 
@@ -183,6 +195,22 @@ int qcs_normalize(qcs_register* r);
 /* <a|b> = sum conj(a_s) b_s in FP64 (stateFidelity :527-534 and
  * ExpectationValue :646-660). a and b may differ in precision. */
 int qcs_inner(qcs_register* a, qcs_register* b, double* re_im);
+/* <r|v> = sum conj(r_s) v_s for a host vector v of 2^n interleaved
+ * doubles, FP64 (stateFidelity, QubitRegister.h:527-534, is its |.|^2).
+ * v streams through the transfer buffer; no second register is made. */
+int qcs_inner_host(qcs_register* r, const double* amps, uint64_t n_amps,
+                   double* re_im);
+/* Exact expectation values of Pauli strings, FP64: out[t] = <psi|P_t|psi>
+ * for P_t = i^{|x_t & z_t|} X^{x_t} Z^{z_t} (qubit q: X if bit q of x_t
+ * only, Z if bit q of z_t only, Y if both, identity if neither). This is
+ * the exact value of what VQE.h:116-142 estimates from 10000 samples after
+ * its change of basis (PauliString.h's Z default included: set its z bit),
+ * and the Pauli-string special case of ExpectationValue
+ * (QubitRegister.h:646-660). Terms sharing an x mask are evaluated in one
+ * read-only pass over the state (up to 32 per pass). */
+int qcs_pauli_expectation(qcs_register* r, uint32_t nterms,
+                          const uint64_t* xmask, const uint64_t* zmask,
+                          double* out);
 /* max_s |a_s - b_s| and <a|b>, for error reports between precisions. */
 int qcs_compare(qcs_register* a, qcs_register* b, double* max_abs,
                 double* inner_re_im);

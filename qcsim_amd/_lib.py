@@ -54,6 +54,7 @@ SIGNATURES = {
     "qcs_apply_dense": (_I, [_P, _U32, _PU32, _PD]),
     "qcs_apply_circuit": (_I, [_P, C.POINTER(QcsGate), _U32, _U32]),
     "qcs_apply_diagonal": (_I, [_P, _PD, _U64]),
+    "qcs_apply_matrix": (_I, [_P, _PD, _U64]),
     "qcs_fuse": (_I, [C.POINTER(QcsGate), _U32, _U32, C.POINTER(QcsGate), C.POINTER(_U32)]),
     "qcs_program_create": (_I, [_P, C.POINTER(QcsGate), _U32, _U32, _I, C.POINTER(_P)]),
     "qcs_program_run": (_I, [_P, _P]),
@@ -69,6 +70,8 @@ SIGNATURES = {
     "qcs_normalize": (_I, [_P]),
     "qcs_inner": (_I, [_P, _P, _PD]),
     "qcs_compare": (_I, [_P, _P, _PD, _PD]),
+    "qcs_inner_host": (_I, [_P, _PD, _U64, _PD]),
+    "qcs_pauli_expectation": (_I, [_P, _U32, _PU64, _PU64, _PD]),
     "qcs_ising_energy": (_I, [_P, _U32, _PU32, _PU32, _PD, _PD, _PD]),
     "qcs_autotune": (_I, [_P, _U32, _PD]),
     "qcs_autotune_variants": (_I, []),

@@ -47,17 +47,6 @@ int check_launch(qcs_register* r, const char* what) {
 int qft_sequence(qcs_register* r, uint32_t sq, uint32_t eq, int do_swap, int inverse);
 int fuse(const qcs_gate* gates, uint32_t n, uint32_t width, std::vector<qcs_gate>& out);
 
-// conversions for complex64 registers
-__global__ void k_d2f(const double2* __restrict__ in, float2* __restrict__ out, uint64_t n) {
-    const uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
-    if (i < n) out[i] = make_float2(float(in[i].x), float(in[i].y));
-}
-__global__ void k_f2d(const float2* __restrict__ in, double2* __restrict__ out, uint64_t n) {
-    const uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
-    if (i < n) out[i] = make_double2(double(in[i].x), double(in[i].y));
-}
-
-
 int stage_upload(qcs_register* r, const void* src, size_t bytes, const void** dev) {
     if (bytes > r->mat_bytes) return set_error(QCS_EINVAL, "parameter block too large");
     if (r->sink) {  // capturing a program: the program owns the parameter memory
@@ -122,6 +111,7 @@ int qcs_create(uint32_t nqubits, int precision, qcs_register** out) {
         if (r->chunk) hipFree(r->chunk);
         if (r->dout) hipFree(r->dout);
         if (r->mat) hipFree(r->mat);
+        if (r->xfer) hipFree(r->xfer);
         if (r->hmat) hipHostFree(r->hmat);
         if (r->hpin) hipHostFree(r->hpin);
         for (hipEvent_t ev : r->mat_ev)
@@ -135,7 +125,7 @@ int qcs_create(uint32_t nqubits, int precision, qcs_register** out) {
         return fail(QCS_ENOMEM, "hipMalloc of the state failed");
     }
     r->mat_bytes = sizeof(double) * 64 * 64;  // real embedding of a 32x32 complex block
-    if (hipMalloc(reinterpret_cast<void**>(&r->red), sizeof(double) * (kRedBlocks * 3 + 64)) != hipSuccess ||
+    if (hipMalloc(reinterpret_cast<void**>(&r->red), sizeof(double) * kRedDoubles) != hipSuccess ||
         hipMalloc(reinterpret_cast<void**>(&r->chunk), sizeof(double) * (nchunks + (nchunks + 2047) / 2048)) !=
             hipSuccess ||
         hipMalloc(reinterpret_cast<void**>(&r->dout), sizeof(uint64_t) * 64) != hipSuccess ||
@@ -172,6 +162,7 @@ int qcs_destroy(qcs_register* r) {
     hipFree(r->chunk);
     hipFree(r->dout);
     hipFree(r->mat);
+    if (r->xfer) hipFree(r->xfer);
     hipHostFree(r->hmat);
     hipHostFree(r->hpin);
     for (hipEvent_t ev : r->mat_ev)
@@ -218,23 +209,7 @@ int qcs_set_state(qcs_register* r, const double* amps, uint64_t n_amps) {
             return set_error(QCS_EDEVICE, "upload failed");
         return hipStreamSynchronize(r->stream) == hipSuccess ? QCS_OK : set_error(QCS_EDEVICE, "sync failed");
     }
-    const uint64_t chunk = r->A < (1ull << 22) ? r->A : (1ull << 22);
-    double2* stage = nullptr;
-    if (hipMalloc(reinterpret_cast<void**>(&stage), chunk * 16) != hipSuccess)
-        return set_error(QCS_ENOMEM, "staging buffer");
-    int e = QCS_OK;
-    for (uint64_t off = 0; off < r->A && !e; off += chunk) {
-        if (hipMemcpyAsync(stage, amps + 2 * off, chunk * 16, hipMemcpyHostToDevice, r->stream) != hipSuccess) {
-            e = set_error(QCS_EDEVICE, "upload failed");
-            break;
-        }
-        hipLaunchKernelGGL(k_d2f, dim3(unsigned((chunk + 255) / 256)), dim3(256), 0, r->stream, stage,
-                           static_cast<float2*>(r->psi) + off, chunk);
-        e = check_launch(r, "k_d2f");
-    }
-    hipStreamSynchronize(r->stream);
-    hipFree(stage);
-    return e;
+    return convert_upload(r, amps);
 }
 
 int qcs_get_state(qcs_register* r, double* amps, uint64_t n_amps) {
@@ -245,23 +220,7 @@ int qcs_get_state(qcs_register* r, double* amps, uint64_t n_amps) {
             return set_error(QCS_EDEVICE, "download failed");
         return hipStreamSynchronize(r->stream) == hipSuccess ? QCS_OK : set_error(QCS_EDEVICE, "sync failed");
     }
-    const uint64_t chunk = r->A < (1ull << 22) ? r->A : (1ull << 22);
-    double2* stage = nullptr;
-    if (hipMalloc(reinterpret_cast<void**>(&stage), chunk * 16) != hipSuccess)
-        return set_error(QCS_ENOMEM, "staging buffer");
-    int e = QCS_OK;
-    for (uint64_t off = 0; off < r->A && !e; off += chunk) {
-        hipLaunchKernelGGL(k_f2d, dim3(unsigned((chunk + 255) / 256)), dim3(256), 0, r->stream,
-                           static_cast<const float2*>(r->psi) + off, stage, chunk);
-        e = check_launch(r, "k_f2d");
-        if (e) break;
-        if (hipMemcpyAsync(amps + 2 * off, stage, chunk * 16, hipMemcpyDeviceToHost, r->stream) != hipSuccess)
-            e = set_error(QCS_EDEVICE, "download failed");
-        hipStreamSynchronize(r->stream);
-    }
-    hipStreamSynchronize(r->stream);
-    hipFree(stage);
-    return e;
+    return convert_download(r, amps);
 }
 
 int qcs_get_amplitude(qcs_register* r, uint64_t idx, double* re_im) {
@@ -426,17 +385,27 @@ int qcs_apply_circuit(qcs_register* r, const qcs_gate* gates, uint32_t n, uint32
 int qcs_apply_diagonal(qcs_register* r, const double* diag, uint64_t n) {
     QCS_REQUIRE(r && diag, "null argument");
     QCS_REQUIRE(n == r->A, "diagonal size mismatch");
-    double* d = nullptr;
-    if (hipMallocAsync(reinterpret_cast<void**>(&d), n * 16, r->stream) != hipSuccess)
-        return set_error(QCS_ENOMEM, "diagonal buffer");
-    // `diag` is pageable caller memory: wait for the kernel before returning
-    // so the caller may release it.
-    int e = hipMemcpyAsync(d, diag, n * 16, hipMemcpyHostToDevice, r->stream) == hipSuccess
-                ? kdiag_full(r, d)
-                : set_error(QCS_EDEVICE, "diagonal upload failed");
-    hipFreeAsync(d, r->stream);
-    if (hipStreamSynchronize(r->stream) != hipSuccess && !e) e = set_error(QCS_EDEVICE, "diagonal sync failed");
-    return e;
+    return apply_diagonal_host(r, diag);
+}
+
+int qcs_apply_matrix(qcs_register* r, const double* mat, uint64_t dim) {
+    QCS_REQUIRE(r && mat, "null argument");
+    QCS_REQUIRE(dim == r->A, "operator size mismatch");
+    return apply_matrix_host(r, mat);
+}
+
+int qcs_inner_host(qcs_register* r, const double* amps, uint64_t n_amps, double* re_im) {
+    QCS_REQUIRE(r && amps && re_im, "null argument");
+    QCS_REQUIRE(n_amps == r->A, "state size mismatch");
+    return inner_host(r, amps, re_im);
+}
+
+int qcs_pauli_expectation(qcs_register* r, uint32_t nterms, const uint64_t* xmask, const uint64_t* zmask,
+                          double* out) {
+    QCS_REQUIRE(r, "null register");
+    if (nterms == 0) return QCS_OK;
+    QCS_REQUIRE(xmask && zmask && out, "null argument");
+    return pauli_expectation(r, nterms, xmask, zmask, out);
 }
 
 int qcs_qft(qcs_register* r, uint32_t sq, uint32_t eq, int do_swap, int inverse) {

@@ -110,6 +110,40 @@ __device__ __forceinline__ cv<T> cmul(T ar, T ai, cv<T> x) {
     return r;
 }
 
+// FP64 reductions over waves of 64 and blocks of 256 (fixed order).
+namespace qcs {
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Block (256 threads = 4 waves) sum; result valid in thread 0.
+__device__ __forceinline__ double block_sum256(double v, double* lds4) {
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) lds4[w] = v;
+    __syncthreads();
+    double t = 0;
+    if (threadIdx.x == 0) t = ((lds4[0] + lds4[1]) + lds4[2]) + lds4[3];
+    __syncthreads();
+    return t;
+}
+
+template <typename T>
+__device__ __forceinline__ double norm2(cv<T> a) {
+    const double x = double(a.x), y = double(a.y);
+    return x * x + y * y;
+}
+
+}  // namespace qcs
+
 // Device parameter space of a program being captured: uploads of dense
 // matrices go to dev + used (copied from the host mirror once the capture
 // ends) instead of the staging ring.
@@ -127,7 +161,7 @@ struct qcs_register {
     int prec = QCS_COMPLEX128;
     int device = 0;
     void* psi = nullptr;        // 2^n amplitudes
-    double* red = nullptr;      // reduction partials (kRedBlocks * 4 doubles)
+    double* red = nullptr;      // reduction partials (kRedDoubles)
     double* chunk = nullptr;    // per-chunk CDF (A / kChunk doubles, >= 1)
     uint64_t* dout = nullptr;   // search results / small device outputs
     // Parameter staging ring for dense matrices: kMatSlots slots of
@@ -145,6 +179,12 @@ struct qcs_register {
     uint64_t launches = 0;
     bool dense_no_mfma = false;  // route dense blocks to the non-MFMA kernel
     ParamSink* sink = nullptr;   // set while a program is captured
+    // Transfer buffer for host vectors streamed through the device in
+    // chunks (complex64 uploads/downloads, diagonal operators, inner
+    // products with a host vector, dense operator panels): allocated on
+    // first use, at most kXferBytes, kept until the register is destroyed.
+    void* xfer = nullptr;
+    size_t xfer_bytes = 0;
 
     size_t amp_bytes() const { return prec == QCS_COMPLEX128 ? 16 : 8; }
 };
@@ -157,6 +197,15 @@ namespace qcs {
 constexpr uint64_t kMaxBlocks = 1ull << 22;
 constexpr int kRedBlocks = 4096;     // fixed grid of reductions (deterministic)
 constexpr int kRedThreads = 256;
+// Pauli-string expectations: at most kPauliTerms terms per pass, partials
+// of at most kPauliBlocks blocks.
+constexpr int kPauliTerms = 32;
+constexpr int kPauliBlocks = 2048;
+// reduction scratch per register: kRedBlocks * 3 partials, or the Pauli
+// partials, then 64 result slots
+constexpr int kRedDoubles = (kRedBlocks * 3 > kPauliTerms * kPauliBlocks ? kRedBlocks * 3 : kPauliTerms * kPauliBlocks) + 64;
+constexpr int kRedResult = kRedDoubles - 64;  // first result slot
+constexpr size_t kXferBytes = size_t(64) << 20;
 constexpr uint32_t kChunkLog = 11;   // CDF chunk = 2048 amplitudes
 constexpr int kScanThreads = 256;
 
@@ -216,5 +265,16 @@ int rising(qcs_register* r, uint32_t ne, const uint32_t* ei, const uint32_t* ej,
 
 // qft.hip
 int qft(qcs_register* r, uint32_t sq, uint32_t eq, int do_swap, int inverse);
+
+// expect_kernels.hip
+// Makes r->xfer at least `bytes` long (at most kXferBytes).
+int ensure_xfer(qcs_register* r, size_t bytes);
+int fetch(qcs_register* r, const double* dev, int n, double* host);
+int pauli_expectation(qcs_register* r, uint32_t nterms, const uint64_t* xmask, const uint64_t* zmask, double* out);
+int inner_host(qcs_register* r, const double* amps, double* re_im);
+int apply_diagonal_host(qcs_register* r, const double* diag);
+int apply_matrix_host(qcs_register* r, const double* mat);
+int convert_upload(qcs_register* r, const double* amps);
+int convert_download(qcs_register* r, double* amps);
 
 }  // namespace qcs

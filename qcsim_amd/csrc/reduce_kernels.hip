@@ -20,36 +20,6 @@
 
 namespace qcs {
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    return v;
-}
-
-// Block (256 threads = 4 waves) sum; result valid in thread 0.
-__device__ __forceinline__ double block_sum256(double v, double* lds4) {
-    v = wave_sum(v);
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) lds4[w] = v;
-    __syncthreads();
-    double t = 0;
-    if (threadIdx.x == 0) t = ((lds4[0] + lds4[1]) + lds4[2]) + lds4[3];
-    __syncthreads();
-    return t;
-}
-
-template <typename T>
-__device__ __forceinline__ double norm2(cv<T> a) {
-    const double x = double(a.x), y = double(a.y);
-    return x * x + y * y;
-}
-
 // Sum of |psi_s|^2 over s = insert_run(o, first, len) | val.
 template <typename T>
 __global__ __launch_bounds__(kRedThreads) void k_sum_sq(const cv<T>* __restrict__ psi, uint64_t norb,
@@ -409,7 +379,7 @@ static int red_blocks(uint64_t work) {
     return int(b);
 }
 
-static int fetch(qcs_register* r, const double* dev, int n, double* host) {
+int fetch(qcs_register* r, const double* dev, int n, double* host) {
     if (hipMemcpyAsync(r->hpin, dev, sizeof(double) * n, hipMemcpyDeviceToHost, r->stream) != hipSuccess)
         return set_error(QCS_EDEVICE, "hipMemcpyAsync D2H failed");
     if (hipStreamSynchronize(r->stream) != hipSuccess) return set_error(QCS_EDEVICE, "hipStreamSynchronize failed");
@@ -440,7 +410,7 @@ static int rsum_sq_t(qcs_register* r, uint32_t first, uint32_t len, uint64_t val
                            static_cast<const cv<T>*>(r->psi), norb, first, len, val, r->red);
         if ((e = check_launch(r, "k_sum_sq"))) return e;
     }
-    double* res = r->red + kRedBlocks * 3;
+    double* res = r->red + kRedResult;
     hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(kRedThreads), 0, r->stream, r->red, nb, 1, res);
     e = check_launch(r, "k_final_sum");
     if (e) return e;
@@ -459,7 +429,7 @@ static int rinner_t(qcs_register* a, qcs_register* b, double* re_im, double* max
                        static_cast<const cv<Ta>*>(a->psi), static_cast<const cv<Tb>*>(b->psi), a->A, a->red);
     int e = check_launch(a, "k_inner");
     if (e) return e;
-    double* res = a->red + kRedBlocks * 3;
+    double* res = a->red + kRedResult;
     hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(kRedThreads), 0, a->stream, a->red, nb, 3, res);
     hipLaunchKernelGGL(k_final_max, dim3(1), dim3(64), 0, a->stream, a->red, nb, res);
     e = check_launch(a, "k_final_sum");
@@ -605,7 +575,7 @@ static int rising_t(qcs_register* r, uint32_t ne, const uint32_t* ei, const uint
                        r->A, ia, dh, dij, dij + ne, dh + r->n, r->red);
     int e = check_launch(r, "k_ising");
     if (e) return e;
-    double* res = r->red + kRedBlocks * 3;
+    double* res = r->red + kRedResult;
     hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(kRedThreads), 0, r->stream, r->red, nb, 1, res);
     e = check_launch(r, "k_final_sum");
     hipFreeAsync(d, r->stream);

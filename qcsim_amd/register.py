@@ -141,14 +141,33 @@ class QubitRegister:
         check(lib().qcs_set_state(self._h, _dptr(v.view(np.float64)), self.NrBasisStates))
 
     def stateFidelity(self, state) -> float:
+        """|<reg|state>|^2 (QubitRegister.h:527-534); the host vector streams
+        through the device, no second register is made."""
         v = np.ascontiguousarray(state, dtype=np.complex128)
         if v.shape[0] != self.NrBasisStates:
             return 0.0
-        other = QubitRegister(self.NrQubits, precision="c128")
-        other.setRegisterStorageFastNoNormalize(v)
-        z = self.inner(other)
-        other.close()
-        return abs(z) ** 2
+        return abs(self.inner_host(v)) ** 2
+
+    def inner_host(self, vals) -> complex:
+        """<reg|vals> for a host vector, FP64."""
+        v = np.ascontiguousarray(vals, dtype=np.complex128)
+        out = np.zeros(2)
+        check(lib().qcs_inner_host(self._h, _dptr(v.view(np.float64)), v.shape[0], _dptr(out)))
+        return complex(out[0], out[1])
+
+    def pauli_expectation(self, xmask, zmask) -> np.ndarray:
+        """Exact <psi|P_t|psi> for Pauli strings given by masks (qubit q is X
+        for bit q of x only, Z for z only, Y for both): qcs_pauli_expectation."""
+        x = np.ascontiguousarray(xmask, dtype=np.uint64)
+        z = np.ascontiguousarray(zmask, dtype=np.uint64)
+        assert x.shape == z.shape and x.ndim == 1
+        out = np.zeros(x.shape[0])
+        check(
+            lib().qcs_pauli_expectation(
+                self._h, x.shape[0], x.ctypes.data_as(_PU64), z.ctypes.data_as(_PU64), _dptr(out)
+            )
+        )
+        return out
 
     def inner(self, other: "QubitRegister") -> complex:
         out = np.zeros(2)
@@ -207,14 +226,24 @@ class QubitRegister:
         return Program(self, gates, fuse_width, adjoint)
 
     def ApplyOperatorMatrix(self, m):
+        """psi <- m psi (QubitRegister.h:499-505) on the device: one pass for a
+        diagonal m, else a dense FP64 product streamed in row panels."""
         m = np.asarray(m, dtype=np.complex128)
+        assert m.shape == (self.NrBasisStates, self.NrBasisStates)
         if np.count_nonzero(m - np.diag(np.diagonal(m))) == 0:
             d = np.ascontiguousarray(np.diagonal(m))
             check(lib().qcs_apply_diagonal(self._h, _dptr(d.view(np.float64)), self.NrBasisStates))
         else:
-            self.setRegisterStorageFastNoNormalize(m @ self.getRegisterStorage())
+            mm = np.ascontiguousarray(m)
+            check(lib().qcs_apply_matrix(self._h, _dptr(mm.view(np.float64)), self.NrBasisStates))
         if self._record:
             self._gates.append(G.AppliedGate(m.copy()))
+
+    def apply_diagonal(self, diag):
+        """psi[s] *= diag[s]: ApplyOperatorMatrix's diagonal case without
+        building the 2^n x 2^n matrix."""
+        d = np.ascontiguousarray(diag, dtype=np.complex128)
+        check(lib().qcs_apply_diagonal(self._h, _dptr(d.view(np.float64)), d.shape[0]))
 
     def qft(self, sq=0, eq=None, do_swap=True, inverse=False):
         eq = self.NrQubits - 1 if eq is None else eq
@@ -332,8 +361,18 @@ class QubitRegister:
         self._record = rec
 
     def ExpectationValue(self, gates) -> complex:
+        """<psi|G|psi> for the product G of the recorded gates
+        (QubitRegister.h:646-660). Single-qubit gates that are multiples of
+        Pauli matrices (the reference's TODO at :649) take one read-only
+        FP64 pass; other lists run on a copy of the state."""
         if not gates:
             return 1.0
+        from .pauli import gates_as_pauli
+
+        pc = gates_as_pauli(gates, self.NrQubits)
+        if pc is not None:
+            coef, x, z = pc
+            return complex(coef * self.pauli_expectation([x], [z])[0])
         saved = QubitRegister(self.NrQubits, precision=self.precision)
         check(lib().qcs_copy_state(saved._h, self._h))
         self.ApplyGates(gates)

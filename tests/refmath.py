@@ -63,3 +63,18 @@ def cdf_first(psi, prob, dflt):
     c = np.cumsum(psi.real * psi.real + psi.imag * psi.imag)
     i = int(np.searchsorted(c, prob, side="left"))
     return i if i < c.shape[0] else dflt
+
+
+def dense_pauli(x, z, n):
+    """2^n x 2^n matrix of the Pauli string with masks (x, z): qubit q is X
+    (x bit only), Z (z bit only), Y = [[0, -i], [i, 0]] (both) or I, built
+    by Kronecker products with qubit 0 as the least significant bit."""
+    I2 = np.eye(2, dtype=np.complex128)
+    X2 = np.array([[0, 1], [1, 0]], dtype=np.complex128)
+    Y2 = np.array([[0, -1j], [1j, 0]], dtype=np.complex128)
+    Z2 = np.array([[1, 0], [0, -1]], dtype=np.complex128)
+    m = np.ones((1, 1), dtype=np.complex128)
+    for q in range(n):
+        xb, zb = (x >> q) & 1, (z >> q) & 1
+        m = np.kron(Y2 if xb and zb else X2 if xb else Z2 if zb else I2, m)
+    return m
