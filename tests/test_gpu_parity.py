@@ -465,3 +465,30 @@ def test_layered_circuit_against_the_oracle_24_qubits():
         print(f"width {width}: max-abs {err:.3e}, infidelity {infid:.3e}")
         assert err < 1e-12, (width, err)
         g.close()
+
+
+def test_toffoli_circuit_c64_fused_against_oracle_24_qubits():
+    """The precision workload's circuit (per layer: Haar U(2) on every qubit, CZ /
+    CNOT on a random pairing, one Toffoli / CCZ / Fredkin) at 24 qubits and
+    40 layers: the complex64 register fused to width-5 blocks (FP32 MFMA)
+    against the oracle's FP64 gate-by-gate run from |0...0>. Max-abs and
+    infidelity are printed; bars 1e-6 and 1e-9."""
+    import bench
+
+    n = 24
+    seq = bench.random_toffoli_circuit(n, 40)
+    o = OracleRegister(n)
+    for gate, qs in seq:
+        o.apply_gate(gate, *qs)
+    ref = o.state()
+    o.close()
+    g = QubitRegister(n, precision="c64")
+    g.apply_circuit(seq, fuse_width=5)
+    psi = g.getRegisterStorage()
+    err = float(np.max(np.abs(psi - ref)))
+    infid = max(0.0, 1 - abs(np.vdot(ref, psi)) ** 2 / (np.vdot(psi, psi).real * np.vdot(ref, ref).real))
+    print(f"24 qubits, {len(seq)} gates, complex64 fused width 5 vs oracle: max-abs {err:.3e}, infidelity {infid:.3e}, "
+          f"max |amplitude| {np.max(np.abs(ref)):.3e}")
+    assert err < 1e-6
+    assert infid < 1e-9
+    g.close()

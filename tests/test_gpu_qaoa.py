@@ -72,3 +72,55 @@ def test_maxcut_known_answer_with_plain_optimiser():
     pick = int(np.argmin(energies * probs))
     mins = set(np.nonzero(np.abs(energies - energies.min()) <= 1e-9)[0].tolist())
     assert pick in mins, (pick, mins)
+
+
+@pytest.mark.parametrize("tile_bits,group", [(12, 6), (11, 5)])
+def test_config_sizes_match_oracle(tile_bits, group):
+    """The batched-QAOA workload's largest instances at its depth: n = 20, 22, 24
+    (12 register passes per evaluation with width-6 mixer passes at tile 12)
+    at p = 4, states and energies against the oracle's gate-by-gate Exec
+    (IsingModel.h:485-567) and exact energy (:106-134)."""
+    insts = make_batch_instances(77 + tile_bits, [20, 22, 24])
+    p = 4
+    rng = np.random.default_rng(6)
+    G = rng.uniform(0, np.pi, (len(insts), p))
+    B = rng.uniform(0, np.pi, (len(insts), p))
+    batch = QaoaBatch(insts, p, tile_bits=tile_bits, group=group)
+    E = batch.evaluate(G, B)
+    for i, inst in enumerate(insts):
+        e_ref, psi_ref = oracle_eval(inst, G[i], B[i])
+        scale = 1 + np.sum(np.abs(inst.J)) + np.sum(np.abs(inst.h))
+        err = float(np.max(np.abs(batch.state(i) - psi_ref)))
+        print(f"n={inst.n}: energy {E[i]:.6f} vs oracle {e_ref:.6f}, state max-abs {err:.2e}")
+        assert abs(E[i] - e_ref) < 2e-5 * scale, (inst.n, E[i], e_ref)
+        assert err < 3e-6, (inst.n, err)
+    batch.close()
+
+
+def test_large_uneven_batch_sample_matches_oracle():
+    """A batch of 256 instances with n uniform over 12, 14, ..., 24 at p = 4 (the
+    batched-QAOA workload's mix at a quarter of its count), evaluated in one
+    persistent launch per stage; a fixed sample of 10 instances (the two
+    largest, the two smallest and 6 more) is compared with the oracle."""
+    from bench import qaoa_batch_instances
+
+    insts, _ = qaoa_batch_instances(256, seed=41)
+    assert len({inst.n for inst in insts}) == 7
+    p = 4
+    rng = np.random.default_rng(7)
+    G = rng.uniform(0, np.pi, (len(insts), p))
+    B = rng.uniform(0, np.pi, (len(insts), p))
+    batch = QaoaBatch(insts, p)
+    E = batch.evaluate(G, B)
+    assert np.all(np.isfinite(E))
+    order = np.argsort([inst.n for inst in insts], kind="stable")
+    sample = sorted(set([int(order[-1]), int(order[-2]), int(order[0]), int(order[1])] + list(range(3, 256, 41))))
+    for i in sample:
+        inst = insts[i]
+        e_ref, psi_ref = oracle_eval(inst, G[i], B[i])
+        scale = 1 + np.sum(np.abs(inst.J)) + np.sum(np.abs(inst.h))
+        err = float(np.max(np.abs(batch.state(i) - psi_ref)))
+        print(f"instance {i} (n={inst.n}): energy error {abs(E[i] - e_ref):.2e}, state max-abs {err:.2e}")
+        assert abs(E[i] - e_ref) < 2e-5 * scale, (i, inst.n, E[i], e_ref)
+        assert err < 3e-6, (i, inst.n, err)
+    batch.close()
