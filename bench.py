@@ -65,8 +65,33 @@ def gpu_sweep(n, steps, warmup, seed=7, tune=True):
     reg.synchronize()
     wall = time.perf_counter() - t0
     norm_err = abs(reg.norm2() - 1.0)
+    accuracy = sweep_accuracy(reg, mats, warmup + steps)
     reg.close()
-    return ev_ms, wall, norm_err, tuned
+    return ev_ms, wall, norm_err, tuned, accuracy
+
+
+def sweep_accuracy(reg, mats, k):
+    """The sweep keeps the state a product state: qubit q ends in U_q^k |+>.
+    A second register is built as that product state from |0...0> with one
+    FP64 gate per qubit, V_q = U_q^k H (matrix powers in FP64 on the host),
+    and the two are compared on the device: max-abs amplitude error and
+    infidelity 1 - |<exact|gpu>|^2 of the timed sweep's result."""
+    from qcsim_amd import QubitRegister
+    from qcsim_amd import gates as G
+
+    n = reg.NrQubits
+    ref = QubitRegister(n)
+    h = G.HadamardGate().getRawOperatorMatrix()
+    for q in range(n):
+        v = np.linalg.matrix_power(mats[q].getRawOperatorMatrix(), k) @ h
+        ref.ApplyGate(G.SingleQubitGate(v), q)
+    mx, ip = reg.compare(ref)
+    ref.close()
+    return {
+        "reference": f"closed-form product state (U_q^{k} |+> on every qubit, built with {n} FP64 gates)",
+        "max_abs_error": mx,
+        "infidelity": max(0.0, 1 - abs(ip) ** 2),
+    }
 
 
 def gpu_qft(n, reps=3):
@@ -210,6 +235,91 @@ def gpu_circuit(n, layers, widths=(0, 4, 5)):
         r.close()
     reg.close()
     return out
+
+
+def random_pairing_circuit(n, layers, seed=17):
+    """The CPU-runnable workload's circuit: per layer a Haar U(2) on every
+    qubit, then CNOT or CZ (fair coin) on each pair of a random perfect
+    pairing of the qubits."""
+    from qcsim_amd import gates as G
+
+    rng = np.random.default_rng(seed)
+    seq = []
+    for _ in range(layers):
+        for q in range(n):
+            seq.append((G.SingleQubitGate(G.haar_u2(rng)), [q]))
+        perm = rng.permutation(n)
+        for a, b in zip(perm[0::2], perm[1::2]):
+            seq.append((G.CNOTGate() if rng.random() < 0.5 else G.ControlledZGate(), [int(a), int(b)]))
+    return seq
+
+
+def cdf_margins(psi, probs):
+    """For each prob = 1 - u: the distance to the nearest value of the
+    sequential FP64 cumulative sum of |psi|^2 (the reference's serial scan,
+    QubitRegister.h:619-642), and the outcome that scan gives: the first i
+    with prob <= cdf(i), else MeasureNoCollapse's default 0."""
+    c = np.cumsum(psi.real * psi.real + psi.imag * psi.imag)
+    i = np.searchsorted(c, probs, side="left")
+    out = np.where(i < c.shape[0], i, 0)
+    hi = np.abs(c[np.minimum(i, c.shape[0] - 1)] - probs)
+    lo = np.abs(c[np.clip(i - 1, 0, None)] - probs)
+    return np.minimum(hi, lo), out
+
+
+def gpu_sampling(n=20, layers=40, shots=1024, seed=5):
+    """The CPU-runnable workload on the GPU: the layered pairing circuit
+    (40 layers) and a 20-qubit QFT from |0...0>, then 1024 outcomes from
+    seeded uniforms, each a MeasureNoCollapse on the final state (the
+    search of MeasureAll with prob = 1 - u, without the collapse, so all 1024
+    draws see the same state). The outcomes
+    are compared with the sequential scan on the downloaded state; the
+    minimum distance of a prob to a CDF boundary is stated."""
+    from qcsim_amd import QubitRegister
+
+    seq = random_pairing_circuit(n, layers)
+    u = np.random.default_rng(seed).random(shots)
+    reg = QubitRegister(n)
+    reg.timer_begin()
+    reg.apply_circuit(seq)
+    reg.qft()
+    ms = reg.timer_end()
+    outcomes = np.array([reg.MeasureNoCollapse(u=float(x)) for x in u], dtype=np.uint64)
+    psi = reg.getRegisterStorage()
+    reg.close()
+    margin, seq_out = cdf_margins(psi, 1.0 - u)
+    return {
+        "qubits": n,
+        "layers": layers,
+        "gates": len(seq) + n + n * (n - 1) // 2 + n // 2,
+        "circuit_ms": round(ms, 3),
+        "shots": shots,
+        "outcomes_equal_sequential_scan": bool(np.array_equal(outcomes, seq_out.astype(np.uint64))),
+        "min_cdf_margin": float(margin.min()),
+        "cdf_rounding_bound": float((1 << n) * np.finfo(np.float64).eps),
+    }, seq, u, psi, outcomes
+
+
+def cpu_config0(seq, u, gpu_psi, gpu_outcomes, n=20):
+    """The oracle runs the same circuit, QFT and searches, timed in full; its
+    amplitudes and outcomes are compared with the GPU's."""
+    from oracle.oracle import OracleRegister
+
+    o = OracleRegister(n)
+    t0 = time.perf_counter()
+    for gate, qs in seq:
+        o.apply_gate(gate, *qs)
+    o.qft()
+    outs = np.array([o.measure_no_collapse(float(x)) for x in u], dtype=np.uint64)
+    dt = time.perf_counter() - t0
+    ref = o.state()
+    o.close()
+    return {
+        "s": round(dt, 3),
+        "max_abs_vs_gpu": float(np.max(np.abs(ref - gpu_psi))),
+        "infidelity_vs_gpu": max(0.0, 1 - abs(np.vdot(ref, gpu_psi)) ** 2),
+        "outcomes_identical": bool(np.array_equal(outs, gpu_outcomes)),
+    }
 
 
 def random_toffoli_circuit(n, layers, seed=13):
@@ -371,27 +481,80 @@ def cpu_qaoa_batch(p=4, seed=31):
     return per_n, int(lib().orc_max_threads())
 
 
-def cpu_baseline(n, targets, seed=7):
+def host_info():
+    """CPU model, sockets, cores and memory of this host (lscpu, /proc)."""
+    import subprocess
+
+    info = {}
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in txt.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)"):
+                info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        with open("/proc/meminfo") as f:
+            info["MemTotal_GiB"] = round(int(f.readline().split()[1]) / 2**20, 1)
+    except OSError:
+        pass
+    return info
+
+
+def cpu_baseline(n, targets, seed=7, reps=5, single_targets=(15,), single_reps=3):
+    """The oracle (the reference's loops restated: results buffer + swap,
+    OpenMP) timed per gate: for each target one warm-up gate, then `reps`
+    timed gates; the figure is the median of the per-target medians. Also a
+    1-thread run on `single_targets` and the host's STREAM triad."""
     from oracle.oracle import OracleRegister, lib
     from qcsim_amd import gates as G
 
     rng = np.random.default_rng(seed)
     r = OracleRegister(n)
-    times = []
+    per_target = {}
     for q in targets:
         g = G.SingleQubitGate(G.haar_u2(rng))
-        t0 = time.perf_counter()
+        r.apply_gate(g, q)  # warm-up (first touch of the results buffer included)
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            r.apply_gate(g, q)
+            ts.append(time.perf_counter() - t0)
+        per_target[q] = float(np.median(ts))
+    lib().orc_set_multithreading(r._h, 0)
+    single = {}
+    for q in single_targets:
+        g = G.SingleQubitGate(G.haar_u2(rng))
         r.apply_gate(g, q)
-        times.append(time.perf_counter() - t0)
+        ts = []
+        for _ in range(single_reps):
+            t0 = time.perf_counter()
+            r.apply_gate(g, q)
+            ts.append(time.perf_counter() - t0)
+        single[q] = float(np.median(ts))
     r.close()
-    per_gate = float(np.median(times))
+    cores = int(lib().orc_max_threads())
+    triad_all = float(lib().orc_stream_triad(1 << 27, 5, 0))
+    triad_one = float(lib().orc_stream_triad(1 << 27, 5, 1))
+    per_gate = float(np.median(list(per_target.values())))
+    one = float(np.median(list(single.values())))
+    gb = algorithmic_bytes_1q(n) / 1e9
     return {
         "value": round(1.0 / per_gate, 4),
         "unit": "gates/s",
-        "cores": int(lib().orc_max_threads()),
+        "cores": cores,
         "kind": "port",
-        "sample": f"{n}-qubit complex128 Haar U(2), one gate on each of targets {list(targets)}; median per gate "
-        f"{per_gate * 1e3:.1f} ms ({algorithmic_bytes_1q(n) / per_gate / 1e9:.1f} GB/s algorithmic)",
+        "sample": f"{n}-qubit complex128 Haar U(2) on targets {list(targets)}: one warm-up and {reps} timed gates "
+        f"per target, median of the per-target medians {per_gate * 1e3:.1f} ms ({gb / per_gate:.1f} GB/s "
+        f"algorithmic); 1 thread on targets {list(single_targets)}: {one * 1e3:.0f} ms per gate",
+        "per_target_ms": {int(q): round(t * 1e3, 2) for q, t in per_target.items()},
+        "algorithmic_GBps": round(gb / per_gate, 1),
+        "single_thread": {"gates_per_s": round(1.0 / one, 4), "algorithmic_GBps": round(gb / one, 2),
+                          "per_target_ms": {int(q): round(t * 1e3, 1) for q, t in single.items()}},
+        "stream_triad_GBps": {"all_threads": round(triad_all, 1), "one_thread": round(triad_one, 1)},
+        "flags": "g++ -O3 -fopenmp -mavx2 -ffp-contract=off",
+        "host": host_info(),
     }
 
 
@@ -410,8 +573,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="use the default launch variants")
     ap.add_argument("--circuit-layers", type=int, default=60,
                     help="depth of the 30-qubit layered random circuit leg (0 = skip)")
-    ap.add_argument("--precision-qubits", type=int, default=0,
-                    help="run the complex64-fused versus FP64 leg at this many qubits (32 = full size)")
+    ap.add_argument("--precision-qubits", type=int, default=32,
+                    help="qubits of the complex64-fused versus FP64 leg (0 = skip)")
     ap.add_argument("--precision-layers", type=int, default=80)
     ap.add_argument("--qaoa-instances", type=int, default=1024, help="batched QAOA instances (0 = skip)")
     ap.add_argument("--learned", type=int, default=1, help="learned-proposal QAOA experiment (0 = skip)")
@@ -437,7 +600,7 @@ def main():
         }
         print(json.dumps({"leg": args.only, "result": legs[args.only]()}))
         return
-    ev_ms, wall, norm_err, tuned = gpu_sweep(n, args.steps, args.warmup, tune=not args.no_tune)
+    ev_ms, wall, norm_err, tuned, accuracy = gpu_sweep(n, args.steps, args.warmup, tune=not args.no_tune)
     launches = n * args.steps
     gates_per_s = launches / (ev_ms / 1e3)
     avg_launch_s = ev_ms / 1e3 / launches
@@ -446,6 +609,8 @@ def main():
     progress(f"sweep: {gates_per_s:.1f} gates/s, {achieved:.0f} GB/s")
     qft = gpu_qft(n)
     progress(f"qft: {qft['ms']} ms")
+    sampling, c0_seq, c0_u, c0_psi, c0_out = gpu_sampling()
+    progress(f"sampling leg done: min CDF margin {sampling['min_cdf_margin']:.2e}")
     circuit = gpu_circuit(n, args.circuit_layers) if args.circuit_layers > 0 else None
     progress("circuit leg done")
     prec = gpu_precision_fused_c64(args.precision_qubits, args.precision_layers) if args.precision_qubits else None
@@ -456,8 +621,11 @@ def main():
     progress("learned QAOA leg done")
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(n, [0, 7, 15, 22, 29] if n >= 30 else list(range(0, n, max(1, n // 5))))
+        cpu = cpu_baseline(n, [0, 7, 15, 22, 29] if n >= 30 else list(range(0, n, max(1, n // 5))),
+                           single_targets=(min(15, n - 1),))
         progress("cpu baseline done")
+        sampling["cpu_reference"] = cpu_config0(c0_seq, c0_u, c0_psi, c0_out)
+        progress("cpu reference of the sampling workload done")
         if qaoa:
             per_n, cores = cpu_qaoa_batch()
             est = sum(per_n[k] * v for k, v in qaoa["qubits_count"].items())
@@ -505,6 +673,8 @@ def main():
         "learned_qaoa": learned,
         "host_wall_s": round(wall, 4),
         "norm_drift": norm_err,
+        "accuracy": accuracy,
+        "sampling": sampling,
     }
     print(json.dumps(out))
 

@@ -54,6 +54,7 @@ def lib():
             "orc_ising_state_energy": (C.c_double, [C.c_uint64, C.c_uint32, C.c_uint32, _PU32, _PU32, _PD, _PD]),
             "orc_qaoa_exec": (None, [C.c_void_p, C.c_uint32, _PU32, _PU32, _PD, _PD, C.c_uint32, _PD, _PD, C.c_int]),
             "orc_last_error": (C.c_char_p, []),
+            "orc_stream_triad": (C.c_double, [C.c_uint64, C.c_int, C.c_int]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)

@@ -30,6 +30,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <memory>
 #include <vector>
 
 #include <omp.h>
@@ -579,5 +580,39 @@ void orc_qaoa_exec(void* h, uint32_t ne, const uint32_t* ei, const uint32_t* ej,
 }
 
 const char* orc_last_error(void) { return g_err; }
+
+// Host memory roofline for the CPU baseline: the STREAM triad
+// a[i] = b[i] + s * c[i] (McCalpin's STREAM benchmark, counted as 3 x 8 B
+// per element) over n doubles per array with `threads` OpenMP threads (0:
+// all), first-touched by the same static schedule; best of `reps` passes,
+// in GB/s.
+double orc_stream_triad(uint64_t n, int reps, int threads) {
+    const int saved = omp_get_max_threads();
+    if (threads > 0) omp_set_num_threads(threads);
+    // uninitialised storage: the first touch below places the pages
+    std::unique_ptr<double[]> a(new double[n]), b(new double[n]), c(new double[n]);
+    double* pa = a.get();
+    double* pb = b.get();
+    double* pc = c.get();
+    const int64_t N = int64_t(n);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < N; ++i) {
+        pa[i] = 0;
+        pb[i] = 1;
+        pc[i] = 2;
+    }
+    double best = 0;
+    for (int r = 0; r < reps; ++r) {
+        const double t0 = omp_get_wtime();
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < N; ++i) pa[i] = pb[i] + 3.0 * pc[i];
+        const double dt = omp_get_wtime() - t0;
+        best = std::max(best, 24.0 * double(n) / dt / 1e9);
+    }
+    omp_set_num_threads(saved);
+    volatile double sink = pa[N / 2];
+    (void)sink;
+    return best;
+}
 
 }  // extern "C"
