@@ -194,9 +194,10 @@ def gate_bytes(gate, n, amp=16):
     return 2 * amp * (1 << n) * f
 
 
-def gpu_circuit(n, layers, widths=(0, 4, 5)):
-    """The layered random circuit, unfused and fused; one run each after a
-    warm-up on a smaller circuit of the same kind."""
+def gpu_circuit(n, layers, widths=(0, 4, 5, "auto")):
+    """The layered random circuit, unfused, fused to 4 and 5 qubits and at the
+    width the fusion autotuner picks; one run each after a warm-up on a
+    smaller circuit of the same kind."""
     from qcsim_amd import QubitRegister
 
     from qcsim_amd.tuning import apply_cache
@@ -217,13 +218,15 @@ def gpu_circuit(n, layers, widths=(0, 4, 5)):
         reg.apply_circuit(seq, fuse_width=w)
         ms = reg.timer_end()
         launches = reg.launch_count() - l0
-        key = "unfused" if w == 0 else f"fused{w}"
+        key = "unfused" if w == 0 else ("auto" if w == "auto" else f"fused{w}")
         out[key] = {
             "s": round(ms / 1e3, 3),
             "gates_per_s": round(len(seq) / (ms / 1e3), 1),
             "launches": int(launches),
             "algorithmic_GBps": round(total_bytes / (ms / 1e3) / 1e9, 1),
         }
+        if w == "auto":
+            out[key]["width"] = reg.last_fuse_width()
         states[key] = reg.Clone()
     if "unfused" in states:
         for k, r in states.items():
@@ -443,19 +446,43 @@ def gpu_qaoa_batch(ninst, p=4, steps=3):
 def gpu_learned_qaoa():
     """Learned QAOA proposals: 1e6 (instance, angles, exact energy) rows from the batched
     engine on 16-20 qubits, the 3x256 BF16 surrogate trained by the HIP
-    kernels, top-32 proposals re-scored exactly on 8 new 20-qubit instances
-    against the plain optimiser (qcsim_amd.learned.run_experiment)."""
-    from qcsim_amd.learned import run_experiment
+    kernels, and on 8 new 20-qubit instances the top-32 proposals re-scored
+    in stages of 4 (stop when a stage improves the best by <= 1e-3), one
+    evaluation at the plain optimiser's default start and the plain
+    optimiser as the fallback when no proposal beats that start
+    (qcsim_amd.learned.run_experiment). Headline: every exact evaluation the
+    learned path makes against the plain optimiser's."""
+    from qcsim_amd.learned import LONG_STEP, run_experiment
 
     out = run_experiment()
     rs = out.pop("results")
     out["plain_evals"] = [r["plain_evals"] for r in rs]
-    out["plain_evals_to_gap"] = [r["plain_evals_to_gap"] for r in rs]
-    out["learned_evals_to_gap"] = [r["learned_evals_to_gap"] for r in rs]
-    out["eval_ratio_budget"] = [round(r["eval_ratio_budget"], 2) for r in rs]
+    out["learned_evals"] = [r["learned_evals"] for r in rs]
+    out["eval_ratio"] = [round(r["eval_ratio"], 2) for r in rs]
+    out["learned_fallbacks"] = sum(r["learned_fallback"] for r in rs)
     out["plain_E"] = [round(r["plain_E"], 4) for r in rs]
     out["learned_E"] = [round(r["learned_E"], 4) for r in rs]
     out["ground_E"] = [round(r["ground_E"], 4) for r in rs]
+    out["all_top32"] = {
+        "note": "all 32 proposals re-scored (the literal top-32 count), no staged stop",
+        "evals": 32,
+        "eval_ratio": [round(r["eval_ratio_all_top"], 2) for r in rs],
+        "best_E": [round(r["all_top_E"], 4) for r in rs],
+    }
+    out["to_gap_secondary"] = {
+        "note": "evaluations until the first proposal reaches plain's best + 1e-3: needs plain's result, "
+                "so no deployment can stop on it",
+        "plain": [r["plain_evals_to_gap"] for r in rs],
+        "learned": [r["learned_evals_to_gap"] for r in rs],
+    }
+    out["strengthened_plain"] = {
+        "note": f"the reference loop with step {LONG_STEP} instead of 5e-4 (same gradient, moments, stop rule), "
+                "from the default start, and warm-started from the learned best (count includes the learned path)",
+        "long_E": [round(r["long_E"], 4) for r in rs],
+        "long_evals": [r["long_evals"] for r in rs],
+        "warm_E": [round(r["warm_E"], 4) for r in rs],
+        "warm_evals": [r["warm_evals"] for r in rs],
+    }
     return out
 
 

@@ -5,9 +5,11 @@ by the project's HIP forward/backward/Adam kernels) maps instance statistics
 plus the 2p normalised angles to the energy, normalised by the instance's
 scale S = sum|J| + sum|h|. It is trained on (instance, angles, exact energy)
 tuples that the batched simulator (QaoaBatch) generates. For a new instance
-it scores many random angle sets, and the best `top` of them are re-scored
-exactly on the statevector; the plain optimiser (qaoa.plain_optimize) remains
-the fallback, warm-started from the best proposal.
+it scores many random angle sets; the best `top` of them, in predicted
+order, are re-scored exactly on the statevector a few at a time until a
+stage brings no improvement (deployable_search), and the plain optimiser
+(qaoa.plain_optimize) is the fallback when no proposal beats its default
+starting point.
 
     python -m qcsim_amd.learned --train-rows 1000000 --sizes 16,18,20 --test-n 20
 """
@@ -148,6 +150,59 @@ def learned_search(model: Surrogate, inst: IsingInstance, p: int, rng: np.random
     return float(E[k]), (G[k], B[k]), top
 
 
+def staged_rescore(inst: IsingInstance, p: int, G, B, batch=4, tol=1e-3, factory=None):
+    """Re-score proposals in predicted order, `batch` at a time, and stop after
+    the first batch (beyond the first) that lowers the running best by no
+    more than `tol`. The rule sees only the proposals' own exact energies.
+    Returns (energies re-scored, in order)."""
+    factory = factory or (lambda insts, pp: QaoaBatch(insts, pp))
+    ev = factory([inst] * batch, p)
+    out = []
+    best = math.inf
+    for b0 in range(0, G.shape[0], batch):
+        g, b = G[b0:b0 + batch], B[b0:b0 + batch]
+        m = g.shape[0]
+        if m < batch:  # a last, short batch: pad with copies of its first row
+            g = np.concatenate([g, np.repeat(g[:1], batch - m, axis=0)])
+            b = np.concatenate([b, np.repeat(b[:1], batch - m, axis=0)])
+        E = ev.evaluate(g, b)[:m]
+        out.extend(float(e) for e in E)
+        new_best = min(best, float(E.min()))
+        improved = best - new_best > tol
+        best = new_best
+        if not improved:
+            break
+    ev.close()
+    return np.array(out)
+
+
+def deployable_search(model: Surrogate, inst: IsingInstance, p: int, rng: np.random.Generator, top=32, batch=4,
+                      tol=1e-3, candidates=65536, start=1.0, optimizer=None):
+    """What a deployment spends: the top `top` proposals re-scored in stages
+    (staged_rescore), one exact evaluation at the plain optimiser's default
+    start (gamma = beta = `start`), and the plain optimiser from that start as
+    the fallback when no proposal beats it. Nothing here looks at the plain
+    optimiser's result. Returns a dict with the best energy, its angles and
+    every exact evaluation counted."""
+    factory = lambda insts, pp: QaoaBatch(insts, pp)  # noqa: E731
+    G, B, _ = propose(model, inst, p, rng, candidates, top)
+    E = staged_rescore(inst, p, G, B, batch, tol, factory)
+    k = int(np.argmin(E))
+    one = factory([inst], p)
+    e_start = float(one.evaluate(np.full((1, p), start), np.full((1, p), start))[0])
+    one.close()
+    out = {"E": float(E[k]), "angles": (G[k], B[k]), "rescored": int(E.shape[0]), "evals": int(E.shape[0]) + 1,
+           "fallback": False, "start_E": e_start, "proposals": (G, B)}
+    if E[k] >= e_start:
+        opt = optimizer or plain_optimize
+        e_fb, ang, ev_fb, _ = opt(factory, inst, [start] * p, [start] * p)
+        out["fallback"] = True
+        out["evals"] += ev_fb
+        if e_fb < out["E"]:
+            out["E"], out["angles"] = e_fb, ang
+    return out
+
+
 def evals_to_target(energies, target: float, per_eval: int = 1):
     """Exact evaluations spent until the running best first reaches
     `target` (energies in evaluation order), or None."""
@@ -159,39 +214,66 @@ def evals_to_target(energies, target: float, per_eval: int = 1):
     return None
 
 
+# The strengthened plain baseline: the reference's loop (same gradient,
+# ADAM-W moments, weight decay and stopping rule) with a step 40 times the
+# reference's 5e-4, so that it leaves its starting point; at the reference
+# step the loop stops after 7-9 iterations close to where it started.
+LONG_STEP = 0.02
+
+
+def long_optimize(factory, inst, gamma0, beta0):
+    return plain_optimize(factory, inst, gamma0, beta0, alpha=LONG_STEP, max_iter=3000)
+
+
 def compare_on_instance(model: Surrogate, inst: IsingInstance, p: int, rng: np.random.Generator, top=32,
-                        gap=1e-3, fallback=True):
+                        gap=1e-3, batch=4, strengthened=True):
     """Plain optimiser against learned proposals on one instance.
 
-    plain: the reference loop from gamma = beta = 1 on exact energies; its
-    evaluations to reach its own final best within `gap` count 4p + 1 per
-    iteration. learned: the top proposals re-scored in predicted order; its
-    evaluations to the same target (plain's best + gap) is the rank of the
-    first proposal that reaches it. With `fallback`, the plain loop then runs
-    warm-started from the best proposal, as a deployment would."""
+    plain: the reference loop from gamma = beta = 1 on exact energies, 4p + 1
+    evaluations per iteration. learned (deployable_search): proposals
+    re-scored in stages with the fallback rule; its count is every exact
+    evaluation it makes. Also reported: all `top` proposals re-scored
+    (config's literal count), the evaluations until the first proposal
+    reaches plain's best + gap (a count that needs plain's result, so no
+    deployment could stop on it), and with `strengthened` the larger-step
+    plain loop from the default start and warm-started from the learned
+    best."""
     factory = lambda insts, pp: QaoaBatch(insts, pp)  # noqa: E731
     e_plain, _, ev_plain, hist = plain_optimize(factory, inst, [1.0] * p, [1.0] * p)
     plain_to_gap = evals_to_target(hist, e_plain + gap, 4 * p + 1)
-    G, B, E = rescore(model, inst, p, rng, top)
-    k = int(np.argmin(E))
-    learned_to_gap = evals_to_target(E, e_plain + gap)
+    dep = deployable_search(model, inst, p, rng, top=top, batch=batch, tol=gap)
+    # the same proposals, all re-scored at once
+    G, B = dep["proposals"]
+    ev = factory([inst] * G.shape[0], p)
+    E_all = ev.evaluate(G, B)
+    ev.close()
     out = {
         "n": inst.n,
         "ground_E": float(inst.energies().min()),
         "plain_E": e_plain,
         "plain_evals": ev_plain,
         "plain_evals_to_gap": plain_to_gap,
-        "learned_E": float(E[k]),
-        "learned_evals": top,
-        "learned_evals_to_gap": learned_to_gap,
-        "within_gap": bool(E[k] <= e_plain + gap),
-        "eval_ratio_budget": ev_plain / top,
-        "eval_ratio_to_gap": (plain_to_gap / learned_to_gap) if learned_to_gap else None,
+        "learned_E": dep["E"],
+        "learned_evals": dep["evals"],
+        "learned_rescored": dep["rescored"],
+        "learned_fallback": dep["fallback"],
+        "within_gap": bool(dep["E"] <= e_plain + gap),
+        "eval_ratio": ev_plain / dep["evals"],
+        "all_top_E": float(E_all.min()),
+        "all_top_evals": top,
+        "eval_ratio_all_top": ev_plain / top,
+        "learned_evals_to_gap": evals_to_target(E_all, e_plain + gap),
     }
-    if fallback:
-        e_fb, _, ev_fb, _ = plain_optimize(factory, inst, list(G[k]), list(B[k]))
-        out["fallback_E"] = min(e_fb, float(E[k]))
-        out["fallback_evals"] = top + ev_fb
+    if strengthened:
+        e_long, _, ev_long, _ = long_optimize(factory, inst, [1.0] * p, [1.0] * p)
+        g0, b0 = dep["angles"]
+        e_warm, _, ev_warm, _ = long_optimize(factory, inst, list(g0), list(b0))
+        out.update({
+            "long_E": e_long,
+            "long_evals": ev_long,
+            "warm_E": min(e_warm, dep["E"]),
+            "warm_evals": dep["evals"] + ev_warm,
+        })
     return out
 
 
@@ -217,12 +299,12 @@ def run_experiment(train_rows=1_000_000, train_instances=2048, sizes=(16, 18, 20
         if progress:
             progress(f"test instance {t}: {json.dumps(results[-1])}")
     model.close()
-    ok = [r for r in results if r["within_gap"]]
-    ratios = [r["eval_ratio_to_gap"] for r in ok if r["eval_ratio_to_gap"]]
+    met = [r for r in results if r["within_gap"] and r["eval_ratio"] >= 5]
     return {"rows": int(X.shape[0]), "gen_s": round(t_gen, 1), "gpu_eval_s": round(t_eval, 1),
             "train_s": round(t_train, 1), "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
-            "within_gap": f"{len(ok)}/{len(results)}",
-            "median_eval_ratio_to_gap": float(np.median(ratios)) if ratios else None,
+            "within_gap": f"{sum(r['within_gap'] for r in results)}/{len(results)}",
+            "target_met": f"{len(met)}/{len(results)}",
+            "median_eval_ratio": float(np.median([r["eval_ratio"] for r in results])),
             "results": results}
 
 
