@@ -111,9 +111,16 @@ int qcs_apply_dense(qcs_register* r, uint32_t k, const uint32_t* qubits,
                     const double* mat);
 
 /* A batch of gates in order; fuse_width 0/1 applies them one by one, 2..5
- * runs the fusion pass first (blocks of at most fuse_width qubits). */
+ * runs the fusion pass first (blocks of at most fuse_width qubits), and
+ * QCS_FUSE_AUTO picks the width (0 or 2..5) whose blocks cost least under
+ * the fusion cost table (qcs_autotune_fusion, measured on first use for
+ * this precision and qubit count). No counterpart in the reference, which
+ * applies gates one by one (QubitRegister.h:488-492). */
+#define QCS_FUSE_AUTO 0xFFFFFFFFu
 int qcs_apply_circuit(qcs_register* r, const qcs_gate* gates, uint32_t n,
                       uint32_t fuse_width);
+/* The width the last qcs_apply_circuit on r used (0: unfused). */
+uint32_t qcs_last_fuse_width(const qcs_register* r);
 
 /* The fusion pass alone (host only, no device): rewrites `gates` into at
  * most n blocks of <= width qubits whose product equals the sequence, each
@@ -249,6 +256,30 @@ int qcs_qaoa_get_state(qcs_qaoa* q, uint32_t inst, double* amps);
 int qcs_qaoa_info(qcs_qaoa* q, uint32_t* stages, uint64_t* amplitudes,
                   uint32_t* tile_bits, uint32_t* group);
 
+/* ---- batched VQE energies (caller of the path: VQE.h:78-212) ----------- */
+
+/* Many registers of 1..12 qubits of uneven size, each prepared by
+ * VQE::PauliStringVQE::Ansatz (VQE.h:78-114: Ry / Rz layers and alternating
+ * CNOT ladders from |0...0>, no gates with fewer parameters than qubits)
+ * and scored with the exact FP64 energy sum_t c_t <psi|P_t|psi> of its
+ * Pauli-string Hamiltonian, the value VariationalQuantumEigensolver::
+ * EstimateEnergy (VQE.h:204-212) estimates from sampled parities. Instance
+ * i has nqubits[i] qubits and nterms[i] terms (x masks, z masks and
+ * coefficients concatenated; masks as in qcs_pauli_expectation). One launch
+ * per size class (<= 8, <= 10, <= 12 qubits), complex128 state in LDS. */
+typedef struct qcs_vqe qcs_vqe;
+int qcs_vqe_create(uint32_t ninst, const uint32_t* nqubits, const uint32_t* nterms,
+                   const uint64_t* xmask, const uint64_t* zmask, const double* coef,
+                   qcs_vqe** out);
+int qcs_vqe_destroy(qcs_vqe* v);
+/* nparams[i] parameters per instance, concatenated in params; energy: ninst. */
+int qcs_vqe_evaluate(qcs_vqe* v, const uint32_t* nparams, const double* params,
+                     double* energy);
+int qcs_vqe_last_ms(qcs_vqe* v, float* ms);
+/* The final state of one instance after the last evaluation (2^n
+ * interleaved doubles). */
+int qcs_vqe_get_state(qcs_vqe* v, uint32_t inst, double* amps);
+
 /* ---- MLP surrogate of QAOA energies (no counterpart in the reference) ---- */
 
 /* in -> hidden (x layers, ReLU) -> 1; BF16 weights and activations, FP32
@@ -278,6 +309,15 @@ int qcs_mlp_train(qcs_mlp* n, const float* X, const float* Y, uint64_t rows, uin
  * algorithmic GB/s figures. */
 int qcs_autotune(qcs_register* r, uint32_t reps, double* gbps);
 int qcs_autotune_variants(void);
+/* The fusion cost table of QCS_FUSE_AUTO: times a generic 1-qubit gate and
+ * dense 2..5-qubit blocks (identity matrices: the state keeps its values)
+ * on this register, mean over low, high and spread qubits, and keeps the
+ * figures for its precision and qubit count. ms, if not NULL, receives 6
+ * values: ms[0] = ms[1] = 1-qubit gate, ms[k] = dense k-qubit block. */
+int qcs_autotune_fusion(qcs_register* r, uint32_t reps, double* ms);
+/* Preset / read that table (6 values as above; zeros: not measured). */
+int qcs_set_fusion_costs(int precision, uint32_t n, const double* ms);
+int qcs_get_fusion_costs(int precision, uint32_t n, double* ms);
 /* Read / preset one entry of that table (vid = -1: untuned default), so a
  * tuning result saved by one process can be reused by the next. */
 int qcs_set_tuned_variant(int precision, uint32_t n, uint32_t q, int vid);

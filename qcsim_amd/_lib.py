@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(_HERE, "libqcsim_amd.so")
 QCS_OK, QCS_EINVAL, QCS_ENOMEM, QCS_EDEVICE = 0, 1, 2, 3
 QCS_GATE_DENSE = 0x1
 QCS_COMPLEX128, QCS_COMPLEX64 = 0, 1
+QCS_FUSE_AUTO = 0xFFFFFFFF
 
 # Every symbol include/qcsim_amd.h declares, with its ctypes signature.
 _P = C.c_void_p
@@ -75,6 +76,10 @@ SIGNATURES = {
     "qcs_ising_energy": (_I, [_P, _U32, _PU32, _PU32, _PD, _PD, _PD]),
     "qcs_autotune": (_I, [_P, _U32, _PD]),
     "qcs_autotune_variants": (_I, []),
+    "qcs_autotune_fusion": (_I, [_P, _U32, _PD]),
+    "qcs_set_fusion_costs": (_I, [_I, _U32, _PD]),
+    "qcs_get_fusion_costs": (_I, [_I, _U32, _PD]),
+    "qcs_last_fuse_width": (_U32, [_P]),
     "qcs_set_tuned_variant": (_I, [_I, _U32, _U32, _I]),
     "qcs_get_tuned_variant": (_I, [_I, _U32, _U32]),
     "qcs_qaoa_create": (_I, [_U32, _PU32, _PU32, _PU32, _PU32, _PD, _PD, _U32, C.POINTER(_P)]),
@@ -84,6 +89,11 @@ SIGNATURES = {
     "qcs_qaoa_last_ms": (_I, [_P, C.POINTER(C.c_float)]),
     "qcs_qaoa_get_state": (_I, [_P, _U32, _PD]),
     "qcs_qaoa_info": (_I, [_P, _PU32, _PU64, _PU32, _PU32]),
+    "qcs_vqe_create": (_I, [_U32, _PU32, _PU32, _PU64, _PU64, _PD, C.POINTER(_P)]),
+    "qcs_vqe_destroy": (_I, [_P]),
+    "qcs_vqe_evaluate": (_I, [_P, _PU32, _PD, _PD]),
+    "qcs_vqe_last_ms": (_I, [_P, C.POINTER(C.c_float)]),
+    "qcs_vqe_get_state": (_I, [_P, _U32, _PD]),
     "qcs_mlp_create": (_I, [_U32, _U32, _U32, _U64, C.POINTER(_P)]),
     "qcs_mlp_destroy": (_I, [_P]),
     "qcs_mlp_num_params": (_U64, [_P]),

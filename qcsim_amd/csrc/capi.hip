@@ -177,6 +177,7 @@ int qcs_destroy(qcs_register* r) {
 uint32_t qcs_get_nqubits(const qcs_register* r) { return r ? r->n : 0; }
 int qcs_get_precision(const qcs_register* r) { return r ? r->prec : -1; }
 uint64_t qcs_launch_count(const qcs_register* r) { return r ? r->launches : 0; }
+uint32_t qcs_last_fuse_width(const qcs_register* r) { return r ? r->last_fuse_width : 0; }
 
 int qcs_copy_state(qcs_register* dst, const qcs_register* src) {
     QCS_REQUIRE(dst && src, "null register");
@@ -360,6 +361,8 @@ int qcs_apply_dense(qcs_register* r, uint32_t k, const uint32_t* qubits, const d
 
 int qcs_apply_circuit(qcs_register* r, const qcs_gate* gates, uint32_t n, uint32_t fuse_width) {
     QCS_REQUIRE(r && (gates || n == 0), "null argument");
+    QCS_REQUIRE(fuse_width <= 5 || fuse_width == QCS_FUSE_AUTO, "fusion width must be 0..5 or QCS_FUSE_AUTO");
+    r->last_fuse_width = fuse_width == QCS_FUSE_AUTO ? 0 : fuse_width;
     std::vector<qcs_gate> fused;
     const qcs_gate* seq = gates;
     uint32_t count = n;
@@ -367,8 +370,11 @@ int qcs_apply_circuit(qcs_register* r, const qcs_gate* gates, uint32_t n, uint32
         for (uint32_t i = 0; i < n; ++i)
             for (uint32_t j = 0; j < gates[i].k && j < 5; ++j)
                 QCS_REQUIRE(gates[i].qubits[j] < r->n, "gate %u: qubit out of range", i);
-        const int e = fuse(gates, n, fuse_width, fused);
+        uint32_t picked = fuse_width;
+        const int e = fuse_width == QCS_FUSE_AUTO ? pick_fusion(r, gates, n, fused, &picked)
+                                                  : fuse(gates, n, fuse_width, fused);
         if (e) return e;
+        r->last_fuse_width = picked;
         seq = fused.data();
         count = uint32_t(fused.size());
     }
@@ -422,11 +428,60 @@ static inline uint64_t range_mask(uint32_t first, uint32_t second) {
     return upto - low;
 }
 
+}  // extern "C"
+
+namespace qcs {
+int measure_no_collapse_impl(qcs_register* r, int all, uint32_t first, uint32_t second, double u,
+                             uint64_t* outcome, const ShotDev* dev) {
+    QCS_REQUIRE(r && outcome, "null argument");
+    if (!all) QCS_REQUIRE(first <= second && second < r->n, "measured qubit range out of bounds");
+    // prob = 1 - u (QubitRegister.h:621), formed on the device
+    uint64_t st = 0;
+    const int e = cdf_search(r, 1, &u, 1, 0, 0, &st, 1, dev);
+    if (e) return e;
+    *outcome = all ? st : (st & range_mask(first, second)) >> first;
+    return QCS_OK;
+}
+
+int sample_impl(qcs_register* r, int all, uint32_t first, uint32_t second, uint64_t nshots, const double* u,
+                uint64_t* outcomes, const ShotDev* dev) {
+    QCS_REQUIRE(r && (u || nshots == 0) && (outcomes || nshots == 0), "null argument");
+    if (!all) QCS_REQUIRE(first <= second && second < r->n, "measured qubit range out of bounds");
+    if (nshots == 0) return QCS_OK;
+    const uint64_t mask = all ? ~0ull : range_mask(first, second);
+    const uint32_t shift = all ? 0 : first;
+    if (nshots == 1) {
+        // single-shot shortcut (QubitRegister.h:233-238, 335-340)
+        uint64_t m = 0;
+        const int e = measure_no_collapse_impl(r, all, first, second, u[0], &m, dev);
+        if (e) return e;
+        outcomes[0] = all ? m : (m & mask) >> shift;
+        return QCS_OK;
+    }
+    // truncation point: first i with cdf(i) > 1 - DBL_EPSILON (QubitRegister.h:250-253)
+    const double thr = 1.0 - DBL_EPSILON;
+    uint64_t t = 0;
+    int e = cdf_search(r, 1, &thr, 0, 1, r->A, &t, 1, dev);
+    if (e) return e;
+    const uint64_t len = t >= r->A ? r->A : t + 1;
+    // prob = 1 - u per shot (QubitRegister.h:262), formed on the device
+    e = cdf_search(r, nshots, u, 1, 0, r->A, outcomes, 0, dev);  // the CDF just built
+    if (e) return e;
+    for (uint64_t s = 0; s < nshots; ++s) {
+        const uint64_t v = outcomes[s] < len ? outcomes[s] : len;  // lower_bound on [0, len)
+        outcomes[s] = all ? v : (v & mask) >> shift;
+    }
+    return QCS_OK;
+}
+}  // namespace qcs
+
+extern "C" {
+
 int qcs_measure_all(qcs_register* r, double u, uint64_t* outcome) {
     QCS_REQUIRE(r && outcome, "null argument");
     const double prob = 1. - u;  // QubitRegister.h:171
     uint64_t st = 0;
-    int e = cdf_search(r, 1, &prob, 0, r->A - 1, &st, 1);
+    int e = cdf_search(r, 1, &prob, 0, 0, r->A - 1, &st, 1);
     if (e) return e;
     e = kfill(r, 0, 0);
     if (!e) e = kset_one(r, st, 1, 0);
@@ -439,7 +494,7 @@ int qcs_measure(qcs_register* r, uint32_t first, uint32_t second, double u, uint
     QCS_REQUIRE(first <= second && second < r->n, "measured qubit range out of bounds");
     const double prob = 1. - u;  // QubitRegister.h:210
     uint64_t pstate = 0;
-    int e = cdf_search(r, 1, &prob, 0, 0, &pstate, 1);
+    int e = cdf_search(r, 1, &prob, 0, 0, 0, &pstate, 1);
     if (e) return e;
     const uint64_t mask = range_mask(first, second);
     const uint64_t meas = pstate & mask;
@@ -453,46 +508,12 @@ int qcs_measure(qcs_register* r, uint32_t first, uint32_t second, double u, uint
 
 int qcs_measure_no_collapse(qcs_register* r, int all, uint32_t first, uint32_t second, double u,
                             uint64_t* outcome) {
-    QCS_REQUIRE(r && outcome, "null argument");
-    if (!all) QCS_REQUIRE(first <= second && second < r->n, "measured qubit range out of bounds");
-    const double prob = 1. - u;
-    uint64_t st = 0;
-    const int e = cdf_search(r, 1, &prob, 0, 0, &st, 1);
-    if (e) return e;
-    *outcome = all ? st : (st & range_mask(first, second)) >> first;
-    return QCS_OK;
+    return measure_no_collapse_impl(r, all, first, second, u, outcome, nullptr);
 }
 
 int qcs_sample(qcs_register* r, int all, uint32_t first, uint32_t second, uint64_t nshots, const double* u,
                uint64_t* outcomes) {
-    QCS_REQUIRE(r && (u || nshots == 0) && (outcomes || nshots == 0), "null argument");
-    if (!all) QCS_REQUIRE(first <= second && second < r->n, "measured qubit range out of bounds");
-    if (nshots == 0) return QCS_OK;
-    const uint64_t mask = all ? ~0ull : range_mask(first, second);
-    const uint32_t shift = all ? 0 : first;
-    if (nshots == 1) {
-        // single-shot shortcut (QubitRegister.h:233-238, 335-340)
-        uint64_t m = 0;
-        const int e = qcs_measure_no_collapse(r, all, first, second, u[0], &m);
-        if (e) return e;
-        outcomes[0] = all ? m : (m & mask) >> shift;
-        return QCS_OK;
-    }
-    // truncation point: first i with cdf(i) > 1 - DBL_EPSILON (QubitRegister.h:250-253)
-    const double thr = 1.0 - DBL_EPSILON;
-    uint64_t t = 0;
-    int e = cdf_search(r, 1, &thr, 1, r->A, &t, 1);
-    if (e) return e;
-    const uint64_t len = t >= r->A ? r->A : t + 1;
-    std::vector<double> probs(nshots);
-    for (uint64_t s = 0; s < nshots; ++s) probs[s] = 1. - u[s];
-    e = cdf_search(r, nshots, probs.data(), 0, r->A, outcomes, 0);  // the CDF just built
-    if (e) return e;
-    for (uint64_t s = 0; s < nshots; ++s) {
-        const uint64_t v = outcomes[s] < len ? outcomes[s] : len;  // lower_bound on [0, len)
-        outcomes[s] = all ? v : (v & mask) >> shift;
-    }
-    return QCS_OK;
+    return sample_impl(r, all, first, second, nshots, u, outcomes, nullptr);
 }
 
 int qcs_qubit_probability(qcs_register* r, uint32_t q, double* p) {

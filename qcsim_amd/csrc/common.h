@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <vector>
 
 #include "qcsim_amd.h"
 
@@ -185,6 +186,7 @@ struct qcs_register {
     // first use, at most kXferBytes, kept until the register is destroyed.
     void* xfer = nullptr;
     size_t xfer_bytes = 0;
+    uint32_t last_fuse_width = 0;  // width the last qcs_apply_circuit used
 
     size_t amp_bytes() const { return prec == QCS_COMPLEX128 ? 16 : 8; }
 };
@@ -252,19 +254,36 @@ int kdense(qcs_register* r, uint32_t k, const uint32_t* qubits, const double* ma
 // reduce_kernels.hip (results in FP64)
 int rsum_sq(qcs_register* r, uint32_t first, uint32_t len, uint64_t val, double* out);
 int rinner(qcs_register* a, qcs_register* b, double* re_im, double* max_abs);
+// Device buffers for the shots of a CDF search: `cap` values in, `cap`
+// outcomes out. A search without them allocates its own, stream-ordered.
+struct ShotDev {
+    double* in = nullptr;
+    uint64_t* out = nullptr;
+    uint64_t cap = 0;
+};
 // CDF search: out[s] = first index whose cumulative probability reaches
-// u_host[s] (exceeds it with `strict`), or `dflt`. build_cdf = 0 reuses the
-// chunk CDF built by the previous search on this register, which is only
-// valid if the state has not changed since (the norms and rsum_sq also use
-// that buffer).
-int cdf_search(qcs_register* r, uint64_t nshots, const double* u_host, int strict,
-               uint64_t dflt, uint64_t* out_host, int build_cdf);
+// prob_s (exceeds it with `strict`), or `dflt`, where prob_s = vals_host[s],
+// or 1 - vals_host[s] (computed on the device) with `one_minus`. build_cdf = 0
+// reuses the chunk CDF built by the previous search on this register, which
+// is only valid if the state has not changed since (the norms and rsum_sq
+// also use that buffer). `dev` (may be null) supplies the shot buffers.
+int cdf_search(qcs_register* r, uint64_t nshots, const double* vals_host, int one_minus, int strict,
+               uint64_t dflt, uint64_t* out_host, int build_cdf, const ShotDev* dev = nullptr);
+// qcs_measure_no_collapse / qcs_sample with the shot buffers of `dev`
+// (null: allocated per call)
+int measure_no_collapse_impl(qcs_register* r, int all, uint32_t first, uint32_t second, double u,
+                             uint64_t* outcome, const ShotDev* dev);
+int sample_impl(qcs_register* r, int all, uint32_t first, uint32_t second, uint64_t nshots, const double* u,
+                uint64_t* outcomes, const ShotDev* dev);
 int kcollapse(qcs_register* r, uint64_t mask, uint64_t val, double scale);
 int rising(qcs_register* r, uint32_t ne, const uint32_t* ei, const uint32_t* ej,
            const double* J, const double* h, double* energy);
 
 // qft.hip
 int qft(qcs_register* r, uint32_t sq, uint32_t eq, int do_swap, int inverse);
+
+// autotune.hip: the fused sequence of the cheapest fusion width
+int pick_fusion(qcs_register* r, const qcs_gate* gates, uint32_t n, std::vector<qcs_gate>& out, uint32_t* width);
 
 // expect_kernels.hip
 // Makes r->xfer at least `bytes` long (at most kXferBytes).

@@ -123,7 +123,8 @@ int qcs_program_destroy(qcs_program* p) {
 int qcs_program_create(qcs_register* r, const qcs_gate* gates, uint32_t n, uint32_t fuse_width, int adjoint,
                        qcs_program** out) {
     if (!r || !out || (!gates && n)) return set_error(QCS_EINVAL, "null argument");
-    if (fuse_width == 1 || fuse_width > 5) return set_error(QCS_EINVAL, "fusion width must be 0 or 2..5");
+    if (fuse_width == 1 || (fuse_width > 5 && fuse_width != QCS_FUSE_AUTO))
+        return set_error(QCS_EINVAL, "fusion width must be 0, 2..5 or QCS_FUSE_AUTO");
     *out = nullptr;
     std::vector<qcs_gate> seq(gates, gates + n);
     for (uint32_t i = 0; i < n; ++i) {
@@ -138,7 +139,10 @@ int qcs_program_create(qcs_register* r, const qcs_gate* gates, uint32_t n, uint3
     auto* p = new qcs_program;
     p->r = r;
     if (fuse_width >= 2) {
-        if (const int e = fuse(seq.data(), uint32_t(seq.size()), fuse_width, p->blocks)) {
+        uint32_t picked = fuse_width;
+        const int e = fuse_width == QCS_FUSE_AUTO ? pick_fusion(r, seq.data(), uint32_t(seq.size()), p->blocks, &picked)
+                                                  : fuse(seq.data(), uint32_t(seq.size()), fuse_width, p->blocks);
+        if (e) {
             delete p;
             return e;
         }

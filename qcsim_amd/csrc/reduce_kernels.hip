@@ -264,13 +264,13 @@ __global__ __launch_bounds__(1024) void k_scan_chunks(double* __restrict__ chunk
 template <typename T>
 __global__ __launch_bounds__(kScanThreads) void k_search(const cv<T>* __restrict__ psi, ChunkGeom g,
                                                          const double* __restrict__ chunk, uint64_t nchunks,
-                                                         const double* __restrict__ probs, int strict,
-                                                         uint64_t dflt, uint64_t* __restrict__ out) {
+                                                         const double* __restrict__ probs, int one_minus,
+                                                         int strict, uint64_t dflt, uint64_t* __restrict__ out) {
     __shared__ double lds[4];
     __shared__ double nrm[kNrmLds];
     __shared__ uint64_t sc;
     __shared__ unsigned long long found;
-    const double prob = probs[blockIdx.x];
+    const double prob = one_minus ? 1.0 - probs[blockIdx.x] : probs[blockIdx.x];
     if (threadIdx.x == 0) {
         // first c with chunk[c] >= prob (strict: > prob)
         uint64_t lo = 0, hi = nchunks;
@@ -461,16 +461,9 @@ static ChunkGeom chunk_geom(const qcs_register* r) {
     return g;
 }
 
-// Device scratch for per-shot inputs/outputs (grown on demand).
-struct ShotBuf {
-    double* probs = nullptr;
-    uint64_t* outs = nullptr;
-    uint64_t cap = 0;
-};
-
 template <typename T>
-static int cdf_search_t(qcs_register* r, uint64_t nshots, const double* probs_host, int strict,
-                        uint64_t dflt, uint64_t* out_host, bool build) {
+static int cdf_search_t(qcs_register* r, uint64_t nshots, const double* vals_host, int one_minus, int strict,
+                        uint64_t dflt, uint64_t* out_host, bool build, const ShotDev* dev) {
     const ChunkGeom g = chunk_geom(r);
     const uint64_t nchunks = r->A / g.clen;
     const auto* psi = static_cast<const cv<T>*>(r->psi);
@@ -503,32 +496,43 @@ static int cdf_search_t(qcs_register* r, uint64_t nshots, const double* probs_ho
         e = check_launch(r, "k_add_group_offsets");
         if (e) return e;
     }
-    // per-shot buffers
-    double* dprobs = nullptr;
+    // per-shot buffers: the caller's, or stream-ordered allocations
+    double* dvals = nullptr;
     uint64_t* douts = nullptr;
-    if (hipMallocAsync(reinterpret_cast<void**>(&dprobs), sizeof(double) * nshots, r->stream) != hipSuccess ||
-        hipMallocAsync(reinterpret_cast<void**>(&douts), sizeof(uint64_t) * nshots, r->stream) != hipSuccess)
+    if (dev) {
+        if (nshots > dev->cap) return set_error(QCS_EINVAL, "%llu shots exceed the buffer of %llu",
+                                                (unsigned long long)nshots, (unsigned long long)dev->cap);
+        dvals = dev->in;
+        douts = dev->out;
+    } else if (hipMallocAsync(reinterpret_cast<void**>(&dvals), sizeof(double) * nshots, r->stream) != hipSuccess ||
+               hipMallocAsync(reinterpret_cast<void**>(&douts), sizeof(uint64_t) * nshots, r->stream) != hipSuccess) {
         return set_error(QCS_ENOMEM, "shot buffers");
-    hipMemcpyAsync(dprobs, probs_host, sizeof(double) * nshots, hipMemcpyHostToDevice, r->stream);
-    for (uint64_t s0 = 0; s0 < nshots; s0 += 65535) {
+    }
+    if (hipMemcpyAsync(dvals, vals_host, sizeof(double) * nshots, hipMemcpyHostToDevice, r->stream) != hipSuccess)
+        e = set_error(QCS_EDEVICE, "shot upload failed");
+    for (uint64_t s0 = 0; s0 < nshots && !e; s0 += 65535) {
         const uint64_t cnt = nshots - s0 < 65535 ? nshots - s0 : 65535;
         hipLaunchKernelGGL((k_search<T>), dim3(unsigned(cnt)), dim3(kScanThreads), 0, r->stream, psi, g, r->chunk,
-                           nchunks, dprobs + s0, strict, dflt, douts + s0);
+                           nchunks, dvals + s0, one_minus, strict, dflt, douts + s0);
         e = check_launch(r, "k_search");
-        if (e) return e;
     }
-    hipMemcpyAsync(out_host, douts, sizeof(uint64_t) * nshots, hipMemcpyDeviceToHost, r->stream);
-    hipFreeAsync(dprobs, r->stream);
-    hipFreeAsync(douts, r->stream);
-    if (hipStreamSynchronize(r->stream) != hipSuccess) return set_error(QCS_EDEVICE, "cdf search sync failed");
-    return QCS_OK;
+    if (!e && hipMemcpyAsync(out_host, douts, sizeof(uint64_t) * nshots, hipMemcpyDeviceToHost, r->stream) !=
+                  hipSuccess)
+        e = set_error(QCS_EDEVICE, "shot download failed");
+    if (!dev) {
+        (void)hipFreeAsync(dvals, r->stream);
+        (void)hipFreeAsync(douts, r->stream);
+    }
+    if (hipStreamSynchronize(r->stream) != hipSuccess && !e) e = set_error(QCS_EDEVICE, "cdf search sync failed");
+    return e;
 }
 
-int cdf_search(qcs_register* r, uint64_t nshots, const double* probs_host, int strict, uint64_t dflt,
-               uint64_t* out_host, int build_cdf) {
+int cdf_search(qcs_register* r, uint64_t nshots, const double* vals_host, int one_minus, int strict, uint64_t dflt,
+               uint64_t* out_host, int build_cdf, const ShotDev* dev) {
     if (nshots == 0) return QCS_OK;
-    return r->prec == QCS_COMPLEX128 ? cdf_search_t<double>(r, nshots, probs_host, strict, dflt, out_host, build_cdf)
-                                     : cdf_search_t<float>(r, nshots, probs_host, strict, dflt, out_host, build_cdf);
+    return r->prec == QCS_COMPLEX128
+               ? cdf_search_t<double>(r, nshots, vals_host, one_minus, strict, dflt, out_host, build_cdf, dev)
+               : cdf_search_t<float>(r, nshots, vals_host, one_minus, strict, dflt, out_host, build_cdf, dev);
 }
 
 int kcollapse(qcs_register* r, uint64_t mask, uint64_t val, double scale) {

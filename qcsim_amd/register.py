@@ -14,7 +14,7 @@ import math
 import numpy as np
 
 from . import gates as G
-from ._lib import QCS_COMPLEX64, QCS_COMPLEX128, QcsGate, check, lib
+from ._lib import QCS_COMPLEX64, QCS_COMPLEX128, QCS_FUSE_AUTO, QcsGate, check, lib
 
 _PD = C.POINTER(C.c_double)
 _PU32 = C.POINTER(C.c_uint32)
@@ -216,11 +216,24 @@ class QubitRegister:
             C.memmove(g.mat, m.ctypes.data, m.nbytes)
         return arr
 
-    def apply_circuit(self, gates, fuse_width: int = 0):
+    def apply_circuit(self, gates, fuse_width=0):
+        """The gates in order; fuse_width 0 (none), 2..5 or "auto" (the width
+        with the lowest cost under the measured fusion cost table)."""
         arr = self._gate_array(gates)
-        check(lib().qcs_apply_circuit(self._h, arr, len(gates), fuse_width))
+        w = QCS_FUSE_AUTO if fuse_width == "auto" else int(fuse_width)
+        check(lib().qcs_apply_circuit(self._h, arr, len(gates), w))
 
-    def compile_program(self, gates, fuse_width: int = 0, adjoint: bool = False) -> "Program":
+    def last_fuse_width(self) -> int:
+        return int(lib().qcs_last_fuse_width(self._h))
+
+    def autotune_fusion(self, reps: int = 2) -> np.ndarray:
+        """Measure the fusion cost table for this precision and size: ms per
+        1-qubit gate (entries 0, 1) and per dense k-qubit block (entry k)."""
+        out = np.zeros(6)
+        check(lib().qcs_autotune_fusion(self._h, reps, _dptr(out)))
+        return out
+
+    def compile_program(self, gates, fuse_width=0, adjoint: bool = False) -> "Program":
         """The gate list captured as a graph for this register (qcs_program_*):
         Compute-style replay, or with adjoint=True the Uncompute order."""
         return Program(self, gates, fuse_width, adjoint)
@@ -487,7 +500,8 @@ class Program:
     def __init__(self, reg: QubitRegister, gates, fuse_width: int = 0, adjoint: bool = False):
         arr = QubitRegister._gate_array(gates)
         h = C.c_void_p()
-        check(lib().qcs_program_create(reg._h, arr, len(gates), fuse_width, int(adjoint), C.byref(h)))
+        w = QCS_FUSE_AUTO if fuse_width == "auto" else int(fuse_width)
+        check(lib().qcs_program_create(reg._h, arr, len(gates), w, int(adjoint), C.byref(h)))
         self._h = h
         self._reg = reg
 

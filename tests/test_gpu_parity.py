@@ -492,3 +492,42 @@ def test_toffoli_circuit_c64_fused_against_oracle_24_qubits():
     assert err < 1e-6
     assert infid < 1e-9
     g.close()
+
+
+def test_fusion_width_autotuner():
+    """QCS_FUSE_AUTO: the measured cost table is positive and grows with the
+    block width; the auto-fused layered circuit equals the oracle at 1e-12;
+    and the width follows the table: with dense blocks priced as cheap as a
+    gate wide blocks (4 or 5 qubits) win, with dense blocks priced 100 times a gate no
+    fusion does."""
+    import ctypes as C
+
+    import bench
+    from qcsim_amd._lib import QCS_COMPLEX128, check, lib
+
+    n = 20
+    seq = bench.random_layered_circuit(n, 12)
+    reg = QubitRegister(n)
+    t = reg.autotune_fusion()
+    print("fusion cost table (ms):", t)
+    assert np.all(t > 0) and t[5] >= t[2] * 0.5
+    reg.apply_circuit(seq, fuse_width="auto")
+    w = reg.last_fuse_width()
+    assert w in (0, 2, 3, 4, 5)
+    o = OracleRegister(n)
+    for gate, qs in seq:
+        o.apply_gate(gate, *qs)
+    err = float(np.max(np.abs(reg.getRegisterStorage() - o.state())))
+    print(f"auto width {w}: max-abs vs oracle {err:.2e}")
+    assert err < 1e-12
+    cheap = np.array([1.0, 1.0, 1.0, 1.0, 1.0, 1.0])
+    check(lib().qcs_set_fusion_costs(QCS_COMPLEX128, n, cheap.ctypes.data_as(C.POINTER(C.c_double))))
+    reg.apply_circuit(seq, fuse_width="auto")
+    assert reg.last_fuse_width() >= 4
+    dear = np.array([1.0, 1.0, 100.0, 100.0, 100.0, 100.0])
+    check(lib().qcs_set_fusion_costs(QCS_COMPLEX128, n, dear.ctypes.data_as(C.POINTER(C.c_double))))
+    reg.apply_circuit(seq, fuse_width="auto")
+    assert reg.last_fuse_width() == 0
+    check(lib().qcs_set_fusion_costs(QCS_COMPLEX128, n, t.ctypes.data_as(C.POINTER(C.c_double))))
+    reg.close()
+    o.close()
