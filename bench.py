@@ -325,6 +325,76 @@ def cpu_config0(seq, u, gpu_psi, gpu_outcomes, n=20):
     }
 
 
+def gpu_pauli(n=30, reps=3, seed=9):
+    """Exact Pauli-string expectations on the n-qubit complex128 state: 96
+    terms over 3 x masks (32 Z-only strings, 32 with X/Y on qubits 0 and
+    n-1, 32 with X/Y on every qubit), one read-only FP64 pass per x mask;
+    ms per call, per pass and GB/s of the passes (s 2^n bytes each)."""
+    from qcsim_amd import QubitRegister
+    from qcsim_amd import gates as G
+
+    rng = np.random.default_rng(seed)
+    reg = QubitRegister(n)
+    reg.setToEqualSuperposition()
+    for q in range(n):
+        reg.ApplyGate(G.SingleQubitGate(G.haar_u2(rng)), q)
+    full = (1 << n) - 1
+    xs = [0] * 32 + [1 | (1 << (n - 1))] * 32 + [full] * 32
+    zs = [int(v) for v in rng.integers(0, 1 << n, size=96)]
+    reg.pauli_expectation(xs, zs)  # warm-up
+    reg.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        vals = reg.pauli_expectation(xs, zs)
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    reg.close()
+    passes = 3
+    return {
+        "qubits": n,
+        "terms": len(xs),
+        "passes": passes,
+        "ms_per_call": round(ms, 3),
+        "ms_per_pass": round(ms / passes, 3),
+        "GBps": round(passes * 16 * (1 << n) / (ms / 1e3) / 1e9, 1),
+        "max_abs_value": float(np.max(np.abs(vals))),
+        "timing": "host wall time around the call, results fetched",
+    }
+
+
+def gpu_vqe_batch(ninst=1024, seed=21, reps=5):
+    """Batched VQE: `ninst` registers with n uniform over 4..12 qubits, each a
+    random Hamiltonian of 24 Pauli strings (a quarter of them diagonal) and
+    4n ansatz parameters (two Ry/Rz layers and a CNOT ladder); exact FP64
+    energies in one launch per size class. Median of `reps` evaluations."""
+    from qcsim_amd.vqe import VqeBatch
+
+    rng = np.random.default_rng(seed)
+    insts, params = [], []
+    for _ in range(ninst):
+        n = int(rng.integers(4, 13))
+        xs = [0 if k % 4 == 0 else int(v) for k, v in enumerate(rng.integers(0, 1 << n, size=24))]
+        zs = [int(v) for v in rng.integers(0, 1 << n, size=24)]
+        insts.append((n, (xs, zs, [float(v) for v in rng.normal(size=24)])))
+        params.append(rng.uniform(0, np.pi, size=4 * n))
+    b = VqeBatch(insts)
+    b.evaluate(params)
+    ms = []
+    for _ in range(reps):
+        b.evaluate(params)
+        ms.append(b.last_ms())
+    b.close()
+    t = float(np.median(ms))
+    return {
+        "instances": ninst,
+        "qubits": "4..12",
+        "terms_per_instance": 24,
+        "params_per_instance": "4n",
+        "batch_ms": round(t, 3),
+        "evals_per_s": round(ninst / (t / 1e3), 1),
+        "timing": "HIP events around the launches (parameters uploaded before)",
+    }
+
+
 def random_toffoli_circuit(n, layers, seed=13):
     """Layered circuit with Toffoli-class gates: per layer a Haar U(2) on
     every qubit, CZ or CNOT on a random perfect pairing, and one
@@ -605,7 +675,7 @@ def main():
     ap.add_argument("--precision-layers", type=int, default=80)
     ap.add_argument("--qaoa-instances", type=int, default=1024, help="batched QAOA instances (0 = skip)")
     ap.add_argument("--learned", type=int, default=1, help="learned-proposal QAOA experiment (0 = skip)")
-    ap.add_argument("--only", choices=["qft", "circuit", "dense", "precision", "qaoa", "learned"],
+    ap.add_argument("--only", choices=["qft", "circuit", "dense", "precision", "qaoa", "learned", "pauli", "vqe"],
                     help="run one leg alone and print its JSON (for profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -624,6 +694,8 @@ def main():
             "precision": lambda: gpu_precision_fused_c64(args.precision_qubits or 32, args.precision_layers),
             "qaoa": lambda: gpu_qaoa_batch(args.qaoa_instances),
             "learned": gpu_learned_qaoa,
+            "pauli": lambda: gpu_pauli(n),
+            "vqe": gpu_vqe_batch,
         }
         print(json.dumps({"leg": args.only, "result": legs[args.only]()}))
         return
@@ -636,6 +708,10 @@ def main():
     progress(f"sweep: {gates_per_s:.1f} gates/s, {achieved:.0f} GB/s")
     qft = gpu_qft(n)
     progress(f"qft: {qft['ms']} ms")
+    pauli = gpu_pauli(n)
+    progress(f"pauli leg: {pauli['ms_per_pass']} ms per pass")
+    vqe = gpu_vqe_batch()
+    progress(f"batched VQE leg: {vqe['batch_ms']} ms per batch")
     sampling, c0_seq, c0_u, c0_psi, c0_out = gpu_sampling()
     progress(f"sampling leg done: min CDF margin {sampling['min_cdf_margin']:.2e}")
     circuit = gpu_circuit(n, args.circuit_layers) if args.circuit_layers > 0 else None
@@ -702,6 +778,8 @@ def main():
         "norm_drift": norm_err,
         "accuracy": accuracy,
         "sampling": sampling,
+        "pauli": pauli,
+        "vqe_batch": vqe,
     }
     print(json.dumps(out))
 
