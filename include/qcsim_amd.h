@@ -192,6 +192,16 @@ int qcs_measure_no_collapse(qcs_register* r, int all, uint32_t first,
  * single-shot shortcut (MeasureNoCollapse). */
 int qcs_sample(qcs_register* r, int all, uint32_t first, uint32_t second,
                uint64_t nshots, const double* u, uint64_t* outcomes);
+/* qcs_sample without allocation: the same outcomes, with the per-shot
+ * device buffers in a fixed 1 MiB array per GPU (held until the process
+ * ends) instead of per-call allocations. No counterpart in the reference:
+ * for callers that cannot allocate (code under stream capture, serving
+ * loops, harnesses that forbid allocation calls). At most 65536 shots per
+ * call (QCS_EINVAL beyond). Calls on registers of the same GPU share the
+ * array: overlapping calls on one GPU race and must be serialised. */
+int qcs_sample_noalloc(qcs_register* r, int all, uint32_t first,
+                       uint32_t second, uint64_t nshots, const double* u,
+                       uint64_t* outcomes);
 
 /* GetQubitProbability(q), QubitRegister.h:592-598, Calc:1088-1122. FP64. */
 int qcs_qubit_probability(qcs_register* r, uint32_t q, double* p);

@@ -66,6 +66,7 @@ SIGNATURES = {
     "qcs_measure": (_I, [_P, _U32, _U32, _D, _PU64]),
     "qcs_measure_no_collapse": (_I, [_P, _I, _U32, _U32, _D, _PU64]),
     "qcs_sample": (_I, [_P, _I, _U32, _U32, _U64, _PD, _PU64]),
+    "qcs_sample_noalloc": (_I, [_P, _I, _U32, _U32, _U64, _PD, _PU64]),
     "qcs_qubit_probability": (_I, [_P, _U32, _PD]),
     "qcs_norm2": (_I, [_P, _PD]),
     "qcs_normalize": (_I, [_P]),
