@@ -224,7 +224,7 @@ int qcs_inner_host(qcs_register* r, const double* amps, uint64_t n_amps,
  * its change of basis (PauliString.h's Z default included: set its z bit),
  * and the Pauli-string special case of ExpectationValue
  * (QubitRegister.h:646-660). Terms sharing an x mask are evaluated in one
- * read-only pass over the state (up to 32 per pass). */
+ * read-only pass over the state (up to 16 per pass). */
 int qcs_pauli_expectation(qcs_register* r, uint32_t nterms,
                           const uint64_t* xmask, const uint64_t* zmask,
                           double* out);

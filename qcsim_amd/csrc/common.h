@@ -201,7 +201,7 @@ constexpr int kRedBlocks = 4096;     // fixed grid of reductions (deterministic)
 constexpr int kRedThreads = 256;
 // Pauli-string expectations: at most kPauliTerms terms per pass, partials
 // of at most kPauliBlocks blocks.
-constexpr int kPauliTerms = 32;
+constexpr int kPauliTerms = 16;
 constexpr int kPauliBlocks = 2048;
 // reduction scratch per register: kRedBlocks * 3 partials, or the Pauli
 // partials, then 64 result slots

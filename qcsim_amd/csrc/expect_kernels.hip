@@ -27,44 +27,61 @@ namespace qcs {
 
 constexpr int kExThreads = 256;
 
+// z masks of one pass, passed by value (kernel arguments: scalar registers).
+struct PauliZ {
+    uint64_t z[kPauliTerms];
+};
+
+// One pass for the terms j < nt that share x. The grid stride is a power of
+// two, so an index splits into disjoint bit sets: s = s_thread | s_iter(k),
+// s_thread from the work-item's index and s_iter from the loop counter k
+// alone (insert_run maps disjoint bit sets to disjoint bit sets). Hence
+// sigma_j(s) = sigma_j(s_thread) sigma_j(s_iter): the second factor is the
+// same for the whole wave and is formed on the scalar unit, the vector unit
+// adds v (+-) per term with FP64 FMAs, and the first factor is applied once
+// per work-item at the end.
 template <typename T, int G>
 __global__ __launch_bounds__(kExThreads) void k_pauli(const cv<T>* __restrict__ psi, uint64_t npairs, uint64_t x,
-                                                      uint32_t hbit, uint32_t nt, const uint64_t* __restrict__ zm,
-                                                      uint32_t odd_mask, double* __restrict__ partial) {
-    __shared__ uint64_t sz[G];
+                                                      uint32_t hbit, uint32_t nt, PauliZ Z, uint32_t odd_mask,
+                                                      double* __restrict__ partial) {
     __shared__ double lds[4];
-    if (threadIdx.x < G) sz[threadIdx.x] = threadIdx.x < nt ? zm[threadIdx.x] : 0;
-    __syncthreads();
     double acc[G];
 #pragma unroll
     for (int j = 0; j < G; ++j) acc[j] = 0;
     const uint64_t stride = uint64_t(gridDim.x) * kExThreads;
-    for (uint64_t o = uint64_t(blockIdx.x) * kExThreads + threadIdx.x; o < npairs; o += stride) {
-        double re, im;
-        uint64_t s;
-        if (x == 0) {
-            s = o;
-            const cv<T> v = ldamp<T, true>(psi + s);
-            re = double(v.x) * double(v.x) + double(v.y) * double(v.y);
-            im = 0;
-        } else {
-            s = insert_run(o, hbit, 1);
-            const cv<T> u = ldamp<T, true>(psi + s);
-            const cv<T> w = ldamp<T, true>(psi + (s ^ x));
-            re = double(u.x) * double(w.x) + double(u.y) * double(w.y);  // conj(u) w
-            im = double(u.x) * double(w.y) - double(u.y) * double(w.x);
+    const uint64_t t = uint64_t(blockIdx.x) * kExThreads + threadIdx.x;
+    const uint64_t st = x ? insert_run(t, hbit, 1) : t;
+    const bool live = t < npairs;  // false only when the state is smaller than one grid
+    for (uint64_t k0 = 0; k0 < npairs; k0 += stride) {
+        const uint64_t sk = x ? insert_run(k0, hbit, 1) : k0;  // the same for every work-item
+        double re = 0, im = 0;
+        if (live) {
+            const uint64_t s = st | sk;
+            if (x == 0) {
+                const cv<T> v = ldamp<T, true>(psi + s);
+                re = double(v.x) * double(v.x) + double(v.y) * double(v.y);
+            } else {
+                const cv<T> u = ldamp<T, true>(psi + s);
+                const cv<T> w = ldamp<T, true>(psi + (s ^ x));
+                re = double(u.x) * double(w.x) + double(u.y) * double(w.y);  // conj(u) w
+                im = double(u.x) * double(w.y) - double(u.y) * double(w.x);
+            }
         }
 #pragma unroll
         for (int j = 0; j < G; ++j) {
-            const double v = (odd_mask >> j & 1) ? im : re;
-            acc[j] += (__popcll(s & sz[j]) & 1) ? -v : v;
+            const double sg = (__popcll(sk & Z.z[j]) & 1) ? -1.0 : 1.0;
+            const double a = (odd_mask >> j & 1) ? 0.0 : sg, b = (odd_mask >> j & 1) ? sg : 0.0;
+            acc[j] = fma(re, a, fma(im, b, acc[j]));
         }
     }
 #pragma unroll
+    for (int j = 0; j < G; ++j)
+        if (__popcll(st & Z.z[j]) & 1) acc[j] = -acc[j];
+#pragma unroll
     for (int j = 0; j < G; ++j) {
         if (uint32_t(j) >= nt) break;
-        const double t = block_sum256(acc[j], lds);
-        if (threadIdx.x == 0) partial[uint64_t(blockIdx.x) * nt + j] = t;
+        const double tot = block_sum256(acc[j], lds);
+        if (threadIdx.x == 0) partial[uint64_t(blockIdx.x) * nt + j] = tot;
     }
 }
 
@@ -207,7 +224,6 @@ static int pauli_t(qcs_register* r, uint32_t nterms, const uint64_t* xmask, cons
             it->second.push_back(t);
         }
     }
-    uint64_t* dz = reinterpret_cast<uint64_t*>(r->dout);  // 64 slots: z masks of one pass
     double* res = r->red + kRedResult;
     for (const uint64_t x : xs) {
         const std::vector<uint32_t>& terms = groups[x];
@@ -216,21 +232,21 @@ static int pauli_t(qcs_register* r, uint32_t nterms, const uint64_t* xmask, cons
         for (size_t t0 = 0; t0 < terms.size(); t0 += kPauliTerms) {
             const uint32_t nt = uint32_t(std::min<size_t>(kPauliTerms, terms.size() - t0));
             uint32_t odd = 0;
+            PauliZ Z{};
             for (uint32_t j = 0; j < nt; ++j) {
                 const uint32_t ny = uint32_t(__builtin_popcountll(x & zmask[terms[t0 + j]]));
-                reinterpret_cast<uint64_t*>(r->hpin)[j] = zmask[terms[t0 + j]];
+                Z.z[j] = zmask[terms[t0 + j]];
                 if (ny & 1) odd |= 1u << j;
             }
-            if (hipMemcpyAsync(dz, r->hpin, sizeof(uint64_t) * nt, hipMemcpyHostToDevice, r->stream) != hipSuccess)
-                return set_error(QCS_EDEVICE, "term upload failed");
-            uint64_t nb = (npairs + kExThreads * 8 - 1) / (kExThreads * 8);
-            nb = std::max<uint64_t>(1, std::min<uint64_t>(nb, kPauliBlocks));
+            // a power-of-two grid (the sign split needs a power-of-two stride)
+            uint64_t nb = 1;
+            while (nb < uint64_t(kPauliBlocks) && nb * kExThreads * 8 < npairs) nb <<= 1;
             if (nt <= 8)
                 hipLaunchKernelGGL((k_pauli<T, 8>), dim3(unsigned(nb)), dim3(kExThreads), 0, r->stream,
-                                   static_cast<const cv<T>*>(r->psi), npairs, x, hbit, nt, dz, odd, r->red);
+                                   static_cast<const cv<T>*>(r->psi), npairs, x, hbit, nt, Z, odd, r->red);
             else
                 hipLaunchKernelGGL((k_pauli<T, kPauliTerms>), dim3(unsigned(nb)), dim3(kExThreads), 0, r->stream,
-                                   static_cast<const cv<T>*>(r->psi), npairs, x, hbit, nt, dz, odd, r->red);
+                                   static_cast<const cv<T>*>(r->psi), npairs, x, hbit, nt, Z, odd, r->red);
             if (const int e = check_launch(r, "k_pauli")) return e;
             hipLaunchKernelGGL(k_pauli_final, dim3(1), dim3(kExThreads), 0, r->stream, r->red, uint32_t(nb), nt, res);
             if (const int e = check_launch(r, "k_pauli_final")) return e;

@@ -201,19 +201,67 @@ __device__ __forceinline__ void rx_reg(float2* x, float c, float s) {
             if (!((k >> q) & 1)) rx_pair(x[k], x[k | (1 << q)], c, s);
 }
 
+// Rx(2 beta) up to a real factor, for passes that rotate every qubit of a
+// tile: Rx = c (I - i t X) with t = tan beta when |c| >= |s| (form 1), else
+// Rx = s (u I - i X) with u = cot beta (form 2); |t|, |u| <= 1. Each output
+// component is then one FMA instead of a product and an FMA; the factor c
+// (or s) per qubit is applied once, as `scale` = c^nq (s^nq), folded into
+// the cost phase of the tile.
+struct RxForm {
+    float k;      // t (form 1) or u (form 2)
+    float scale;  // c^nq or s^nq
+    int f1;       // 1: form 1
+};
+
+__device__ __forceinline__ RxForm rx_form(double c, double s, uint32_t nq) {
+    const bool f1 = fabs(c) >= fabs(s);
+    const double f = f1 ? c : s;
+    double sc = 1;
+    for (uint32_t i = 0; i < nq; ++i) sc *= f;
+    return RxForm{float(f1 ? s / c : c / s), float(sc), f1 ? 1 : 0};
+}
+
+template <bool F1>
+__device__ __forceinline__ void rxs_pair(float2& a, float2& b, float k) {
+    const float2 x = a, y = b;
+    if (F1) {  // a - i t b, b - i t a
+        a = make_float2(fmaf(k, y.y, x.x), fmaf(-k, y.x, x.y));
+        b = make_float2(fmaf(k, x.y, y.x), fmaf(-k, x.x, y.y));
+    } else {   // u a - i b, u b - i a
+        a = make_float2(fmaf(k, x.x, y.y), fmaf(k, x.y, -y.x));
+        b = make_float2(fmaf(k, y.x, x.y), fmaf(k, y.y, -x.x));
+    }
+}
+
+template <int G, bool F1>
+__device__ __forceinline__ void rxs_reg_f(float2* x, float k) {
+    constexpr int D = 1 << G;
+#pragma unroll
+    for (int q = 0; q < G; ++q)
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+            if (!((i >> q) & 1)) rxs_pair<F1>(x[i], x[i | (1 << q)], k);
+}
+
+template <int G>
+__device__ __forceinline__ void rxs_reg(float2* x, const RxForm& r) {
+    if (r.f1) rxs_reg_f<G, true>(x, r.k);
+    else rxs_reg_f<G, false>(x, r.k);
+}
+
 enum : int { IN_PLUS = 0, IN_GLOBAL = 1, IN_LDS = 2 };
 
 // Rx(2 beta) on the G tile qubits lo..lo+G-1, each thread (index tid) taking
 // orbits of 2^G elements from the LDS tile into registers.
 template <int G>
-__device__ void rx_round(float2* t, uint32_t tid, uint32_t tb, uint32_t lo, float c, float s) {
+__device__ void rx_round(float2* t, uint32_t tid, uint32_t tb, uint32_t lo, const RxForm& rf) {
     constexpr int D = 1 << G;
     for (uint32_t o = tid; o < (1u << (tb - G)); o += 256) {
         const uint32_t lb = uint32_t(insert_run(o, lo, G));
         float2 y[D];
 #pragma unroll
         for (int k = 0; k < D; ++k) y[k] = t[swz(lb + (uint32_t(k) << lo))];
-        rx_reg<G>(y, c, s);
+        rxs_reg<G>(y, rf);
 #pragma unroll
         for (int k = 0; k < D; ++k) t[swz(lb + (uint32_t(k) << lo))] = y[k];
     }
@@ -230,12 +278,12 @@ __device__ __forceinline__ uint32_t wave_orbit_base(uint32_t tid, uint32_t lo) {
     return ((lane & 15) << (lo == 0 ? 4 : 0)) | ((lane >> 4) << 8) | (w << 10);
 }
 
-__device__ __forceinline__ void rx_wave(float2* t, uint32_t tid, uint32_t lo, float c, float s) {
+__device__ __forceinline__ void rx_wave(float2* t, uint32_t tid, uint32_t lo, const RxForm& rf) {
     const uint32_t lb = wave_orbit_base(tid, lo);
     float2 y[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) y[k] = t[swz(lb | (uint32_t(k) << lo))];
-    rx_reg<4>(y, c, s);
+    rxs_reg<4>(y, rf);
 #pragma unroll
     for (int k = 0; k < 16; ++k) t[swz(lb | (uint32_t(k) << lo))] = y[k];
 }
@@ -336,7 +384,7 @@ __device__ void thread_terms(ThreadTerms& W, double gamma, const QInst& I, const
 // so that the compiler recomputes the LDS addresses per tile instead of
 // holding them all in registers across a run of tiles.
 template <int M>
-__device__ void tile_layer(float2* t, uint32_t tid, uint64_t base, double gamma, float c, float s, const QInst& I,
+__device__ void tile_layer(float2* t, uint32_t tid, uint64_t base, double gamma, const RxForm& rf, const QInst& I,
                            const IsingLds& P, TileCostLds& TC, const ThreadTerms& W, int in_mode,
                            const float2* __restrict__ in, float amp) {
     constexpr uint32_t tb = 8 + M;
@@ -376,6 +424,8 @@ __device__ void tile_layer(float2* t, uint32_t tid, uint64_t base, double gamma,
         e[k] = make_float2(float(x * r), float(y * r));
     }
     float2 ph = expmi(gamma * A);
+    ph.x *= rf.scale;  // the real factor of the tb scaled rotations below
+    ph.y *= rf.scale;
     float2 y[1 << M];
 #pragma unroll
     for (uint32_t m = 0; m < (1u << M); ++m) {
@@ -395,29 +445,30 @@ __device__ void tile_layer(float2* t, uint32_t tid, uint64_t base, double gamma,
         y[gm] = make_float2(wr * v.x - wi * v.y, wr * v.y + wi * v.x);
     }
     // y is the orbit of qubits 8..tb-1: their Rx before the LDS store
-    if constexpr (M > 0) rx_reg<M>(y, c, s);
+    if constexpr (M > 0) rxs_reg<M>(y, rf);
 #pragma unroll
     for (uint32_t m = 0; m < (1u << M); ++m) t[swz(tid + (m << 8))] = y[m];
     __syncthreads();
     if constexpr (M == 4) {
         // wave-local rounds; the caller orders the result before other waves
         // read it
-        rx_wave(t, tid, 4, c, s);
+        rx_wave(t, tid, 4, rf);
         lds_wave_sync();
-        rx_wave(t, tid, 0, c, s);
+        rx_wave(t, tid, 0, rf);
         lds_wave_sync();
     } else {
-        rx_round<4>(t, tid, tb, 4, c, s);
-        rx_round<4>(t, tid, tb, 0, c, s);
+        rx_round<4>(t, tid, tb, 4, rf);
+        rx_round<4>(t, tid, tb, 0, rf);
     }
 }
 
 // A tile unit: one layer on tiles tile0 .. tile0+ntile-1 of an instance,
 // 2^(8+M) amplitudes each, written back to psi.
 template <int M>
-__device__ void tile_run(float2* t, float2* psi, uint64_t tile0, uint32_t ntile, double gamma, float c, float s,
+__device__ void tile_run(float2* t, float2* psi, uint64_t tile0, uint32_t ntile, double gamma, double c, double s,
                          const QInst& I, const IsingLds& P, TileCostLds& TC, bool first, float amp) {
     constexpr uint32_t tb = 8 + M, C = 1u << tb;
+    const RxForm rf = rx_form(c, s, tb);
     ThreadTerms W;
     thread_terms<M>(W, gamma, I, P, TC);
     for (uint32_t j = 0; j < ntile; ++j) {
@@ -425,7 +476,7 @@ __device__ void tile_run(float2* t, float2* psi, uint64_t tile0, uint32_t ntile,
         uint32_t tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
         __builtin_assume(tid < 256u);
-        tile_layer<M>(t, tid, base, gamma, c, s, I, P, TC, W, first ? IN_PLUS : IN_GLOBAL, psi + base, amp);
+        tile_layer<M>(t, tid, base, gamma, rf, I, P, TC, W, first ? IN_PLUS : IN_GLOBAL, psi + base, amp);
         if constexpr (M == 4) {
             // each wave stores the 1024 elements it rotated last
 #pragma unroll
@@ -456,13 +507,13 @@ __device__ void small_run(float2* t, const double* an, uint32_t p, const QInst& 
         uint32_t tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
         __builtin_assume(tid < 256u);
-        tile_layer<M>(t, tid, 0, an[2 * l], float(cb), float(sb), I, P, TC, W, l ? IN_LDS : IN_PLUS, nullptr, amp);
+        tile_layer<M>(t, tid, 0, an[2 * l], rx_form(cb, sb, 8 + M), I, P, TC, W, l ? IN_LDS : IN_PLUS, nullptr, amp);
     }
     __syncthreads();  // the energy pass reads every wave's elements
 }
 
 __device__ __forceinline__ void tile_run_tb(uint32_t tb, float2* t, float2* psi, uint64_t tile0, uint32_t ntile, double gamma,
-                            float c, float s, const QInst& I, const IsingLds& P, TileCostLds& TC, bool first,
+                            double c, double s, const QInst& I, const IsingLds& P, TileCostLds& TC, bool first,
                             float amp) {
     switch (tb) {
         case 8: tile_run<0>(t, psi, tile0, ntile, gamma, c, s, I, P, TC, first, amp); break;
@@ -551,7 +602,7 @@ __global__ __launch_bounds__(256, 4) void k_qaoa_tile(float2* __restrict__ arena
             double sb, cb;
             sincos(an[2 * U.layer + 1], &sb, &cb);
             if (tb >= 8) {
-                tile_run_tb(tb, tile, psi, U.idx, U.ntile, an[2 * U.layer], float(cb), float(sb), I, P, TC,
+                tile_run_tb(tb, tile, psi, U.idx, U.ntile, an[2 * U.layer], cb, sb, I, P, TC,
                             U.layer == 0, amp);
             } else {
                 for (uint32_t j = 0; j < U.ntile; ++j) {

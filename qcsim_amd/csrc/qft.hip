@@ -295,31 +295,151 @@ __global__ __launch_bounds__(256) void k_qft_group5(cv<T>* __restrict__ psi, uin
 // 16 lanes of one access phase hit different banks for every round layout.
 __device__ __forceinline__ uint32_t swz(uint32_t i) { return i ^ ((i >> 4) & 15u); }
 
+// x * exp(i pi J / 32) (conjugate with CONJ) for a root index J known at
+// compile time: J = 0 is the identity, J = 16 is a multiplication by i.
+template <typename T, int J, bool CONJ>
+__device__ __forceinline__ cv<T> mul_root(cv<T> x, const QftRoots& R) {
+    static_assert(J >= 0 && J < (1 << kGroupMax), "root index");
+    if constexpr (J == 0) {
+        return x;
+    } else if constexpr (J == 16) {
+        cv<T> r;
+        r.x = CONJ ? x.y : -x.y;
+        r.y = CONJ ? -x.x : x.x;
+        return r;
+    } else {
+        return mul_ph<T>(x, R.w[J], CONJ);
+    }
+}
+
+// Intra-group phase of target tl on element c (the part of D_(lo+tl+1) that
+// depends on the group bits lo..lo+tl): exp(i pi Gt / 2^(tl+1)),
+// Gt = c & (2^(tl+1) - 1), for elements whose bit tl+1 is set.
+template <typename T, int G, int TL, bool CONJ>
+__device__ __forceinline__ void apply_p(cv<T>* x, const QftRoots& R) {
+#pragma unroll
+    for (int c = 0; c < (1 << G); ++c) {
+        if (!((c >> (TL + 1)) & 1)) continue;
+        constexpr int shift = kGroupMax - 1 - TL;
+        const int Gt = c & ((2 << TL) - 1);
+        switch (Gt << shift) {  // the index is a compile-time constant per unrolled c
+#define QCS_ROOT_CASE(J) \
+    case J: x[c] = mul_root<T, J, CONJ>(x[c], R); break;
+            QCS_ROOT_CASE(0) QCS_ROOT_CASE(1) QCS_ROOT_CASE(2) QCS_ROOT_CASE(3) QCS_ROOT_CASE(4) QCS_ROOT_CASE(5)
+            QCS_ROOT_CASE(6) QCS_ROOT_CASE(7) QCS_ROOT_CASE(8) QCS_ROOT_CASE(9) QCS_ROOT_CASE(10) QCS_ROOT_CASE(11)
+            QCS_ROOT_CASE(12) QCS_ROOT_CASE(13) QCS_ROOT_CASE(14) QCS_ROOT_CASE(15) QCS_ROOT_CASE(16)
+            QCS_ROOT_CASE(17) QCS_ROOT_CASE(18) QCS_ROOT_CASE(19) QCS_ROOT_CASE(20) QCS_ROOT_CASE(21)
+            QCS_ROOT_CASE(22) QCS_ROOT_CASE(23) QCS_ROOT_CASE(24) QCS_ROOT_CASE(25) QCS_ROOT_CASE(26)
+            QCS_ROOT_CASE(27) QCS_ROOT_CASE(28) QCS_ROOT_CASE(29) QCS_ROOT_CASE(30) QCS_ROOT_CASE(31)
+#undef QCS_ROOT_CASE
+            default: break;
+        }
+    }
+}
+
+// The top target's intra-group phase (its bit tl + 1 = G is above the group
+// and set for every element of the orbit when it is set at all):
+// exp(i pi c / 2^G) on every element c.
+template <typename T, int G, bool CONJ>
+__device__ __forceinline__ void apply_p_top(cv<T>* x, const QftRoots& R) {
+#pragma unroll
+    for (int c = 0; c < (1 << G); ++c) {
+        switch (c << (kGroupMax - G)) {
+#define QCS_ROOT_CASE(J) \
+    case J: x[c] = mul_root<T, J, CONJ>(x[c], R); break;
+            QCS_ROOT_CASE(0) QCS_ROOT_CASE(1) QCS_ROOT_CASE(2) QCS_ROOT_CASE(3) QCS_ROOT_CASE(4) QCS_ROOT_CASE(5)
+            QCS_ROOT_CASE(6) QCS_ROOT_CASE(7) QCS_ROOT_CASE(8) QCS_ROOT_CASE(9) QCS_ROOT_CASE(10) QCS_ROOT_CASE(11)
+            QCS_ROOT_CASE(12) QCS_ROOT_CASE(13) QCS_ROOT_CASE(14) QCS_ROOT_CASE(15) QCS_ROOT_CASE(16)
+            QCS_ROOT_CASE(17) QCS_ROOT_CASE(18) QCS_ROOT_CASE(19) QCS_ROOT_CASE(20) QCS_ROOT_CASE(21)
+            QCS_ROOT_CASE(22) QCS_ROOT_CASE(23) QCS_ROOT_CASE(24) QCS_ROOT_CASE(25) QCS_ROOT_CASE(26)
+            QCS_ROOT_CASE(27) QCS_ROOT_CASE(28) QCS_ROOT_CASE(29) QCS_ROOT_CASE(30) QCS_ROOT_CASE(31)
+#undef QCS_ROOT_CASE
+            default: break;
+        }
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ double2 cmul2(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
 // One round of k_qft_lds: targets a.lo .. a.lo+G-1, each thread taking
-// orbits of 2^G chunk elements into registers (the k_qft_group arithmetic).
+// orbits of 2^G chunk elements into registers.
+//
+// The phase of D_(t+1) on an element splits into the part from the group
+// bits lo..t (apply_p) and F_t = exp(i pi Lf / 2^(t+1-sq)) from the bits
+// sq..lo-1 below the group, which the round does not change. F_t multiplies
+// the elements whose bit t+1 is set, and every later operation of the round
+// (forward: Hadamards and phases on bits <= t) keeps that set of elements
+// and commutes with a factor that depends only on bits outside lo..t; so
+// all F_t of the round can be applied at its end as one factor per element:
+// F_t = F^(2^(top-t)) with F = F_top, hence the product is F^m with
+// m = sum over the round's phased targets t with bit t+1 set of 2^(top-t).
+// In the inverse the same argument moves the conjugate factors to the
+// start of the round. Per orbit a table Q[j] = F^(2j + b) (b = the element-
+// independent bit above the group) is built with a few complex products;
+// each element then takes one complex product instead of one per target.
 template <typename T, int G, bool INV>
 __device__ void lds_round(cv<T>* s, uint64_t cbase, uint32_t clog, const QftArgs& a, const double2* __restrict__ tw,
                           const QftRoots& R) {
     constexpr int D = 1 << G;
     const uint32_t norb = 1u << (clog - G);
+    const uint32_t top = a.lo + G - 1;
+    const bool has_top = top < a.eq;  // the top target has a phase (all lower ones do)
+    const uint32_t nlow = a.lo - a.sq;
+    const uint32_t ktop = top + 1 - a.sq;
     for (uint32_t o = threadIdx.x; o < norb; o += 256) {
         const uint32_t lb = uint32_t(insert_run(o, a.lo, G));
         cv<T> x[D];
 #pragma unroll
         for (int c = 0; c < D; ++c) x[c] = s[swz(lb + (uint32_t(c) << a.lo))];
         const uint64_t base = cbase | lb;
+        const bool bt = has_top && ((base >> (top + 1)) & 1);
+        const uint64_t Lf = nlow ? ((base >> a.sq) & ((1ull << nlow) - 1)) : 0;
+        // Q[j] = F^(2j + bt), j < D / 2
+        double2 Q[D / 2];
+        if (Lf) {
+            const double2 F = phase_pi(tw, Lf, ktop);
+            const double2 F2 = cmul2<T>(F, F);
+            Q[0] = bt ? F : make_double2(1.0, 0.0);
+#pragma unroll
+            for (int j = 1; j < D / 2; ++j) Q[j] = cmul2<T>(Q[j - 1], F2);
+        }
+        auto apply_f = [&]() {
+            if (!Lf) return;
+#pragma unroll
+            for (int c = 0; c < D; ++c) {
+                int mc = 0;  // the even part of m: bits 1..G-1 of c, weights 2^(G-1-tl)
+#pragma unroll
+                for (int tl = 0; tl + 1 < G; ++tl) mc += ((c >> (tl + 1)) & 1) << (G - 1 - tl);
+                x[c] = mul_ph<T>(x[c], Q[mc >> 1], INV);
+            }
+        };
         if (!INV) {
-#pragma unroll
-            for (int tl = G - 1; tl >= 0; --tl) {
-                if (a.lo + tl < a.eq) apply_d<T, G>(x, base, a.lo + tl, a, tw, R, false);
-                apply_h<T, G>(x, tl);
+            if (bt) apply_p_top<T, G, false>(x, R);
+            apply_h<T, G>(x, G - 1);
+            if constexpr (G >= 2) {
+                apply_p<T, G, G - 2, false>(x, R);
+                apply_h<T, G>(x, G - 2);
             }
+            if constexpr (G >= 3) {
+                apply_p<T, G, G - 3, false>(x, R);
+                apply_h<T, G>(x, G - 3);
+            }
+            apply_f();
         } else {
-#pragma unroll
-            for (int tl = 0; tl < G; ++tl) {
-                apply_h<T, G>(x, tl);
-                if (a.lo + tl < a.eq) apply_d<T, G>(x, base, a.lo + tl, a, tw, R, true);
+            apply_f();
+            if constexpr (G >= 3) {
+                apply_h<T, G>(x, G - 3);
+                apply_p<T, G, G - 3, true>(x, R);
             }
+            if constexpr (G >= 2) {
+                apply_h<T, G>(x, G - 2);
+                apply_p<T, G, G - 2, true>(x, R);
+            }
+            apply_h<T, G>(x, G - 1);
+            if (bt) apply_p_top<T, G, true>(x, R);
         }
 #pragma unroll
         for (int c = 0; c < D; ++c) s[swz(lb + (uint32_t(c) << a.lo))] = x[c];
