@@ -38,8 +38,9 @@ struct PauliZ {
 // alone (insert_run maps disjoint bit sets to disjoint bit sets). Hence
 // sigma_j(s) = sigma_j(s_thread) sigma_j(s_iter): the second factor is the
 // same for the whole wave and is formed on the scalar unit, the vector unit
-// adds v (+-) per term with FP64 FMAs, and the first factor is applied once
-// per work-item at the end.
+// adds v (+-) per term with FP64 FMAs (one for x = 0, where every term takes
+// the real part, two otherwise), and the first factor is applied once per
+// work-item at the end.
 template <typename T, int G>
 __global__ __launch_bounds__(kExThreads) void k_pauli(const cv<T>* __restrict__ psi, uint64_t npairs, uint64_t x,
                                                       uint32_t hbit, uint32_t nt, PauliZ Z, uint32_t odd_mask,
@@ -67,11 +68,19 @@ __global__ __launch_bounds__(kExThreads) void k_pauli(const cv<T>* __restrict__ 
                 im = double(u.x) * double(w.y) - double(u.y) * double(w.x);
             }
         }
+        if (x == 0) {  // every term even, im = 0: one FMA per term
 #pragma unroll
-        for (int j = 0; j < G; ++j) {
-            const double sg = (__popcll(sk & Z.z[j]) & 1) ? -1.0 : 1.0;
-            const double a = (odd_mask >> j & 1) ? 0.0 : sg, b = (odd_mask >> j & 1) ? sg : 0.0;
-            acc[j] = fma(re, a, fma(im, b, acc[j]));
+            for (int j = 0; j < G; ++j) {
+                const double sg = (__popcll(sk & Z.z[j]) & 1) ? -1.0 : 1.0;
+                acc[j] = fma(re, sg, acc[j]);
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < G; ++j) {
+                const double sg = (__popcll(sk & Z.z[j]) & 1) ? -1.0 : 1.0;
+                const double a = (odd_mask >> j & 1) ? 0.0 : sg, b = (odd_mask >> j & 1) ? sg : 0.0;
+                acc[j] = fma(re, a, fma(im, b, acc[j]));
+            }
         }
     }
 #pragma unroll
