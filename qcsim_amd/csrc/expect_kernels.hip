@@ -27,71 +27,142 @@ namespace qcs {
 
 constexpr int kExThreads = 256;
 
-// z masks of one pass, passed by value (kernel arguments: scalar registers).
-struct PauliZ {
+constexpr uint32_t kFlipBits = 16;  // iteration bits: 2^35 amplitudes over a grid of 2^19 work-items
+
+// One pass's terms, passed by value (kernel arguments, read by scalar loads).
+// Iteration k of a work-item visits s = s_thread | s_iter(g(k)), g the Gray
+// code, so step k (k >= 1) flips exactly iteration bit b = ctz(k): state bit
+// bit[b], and the sign of term j flips by w[b][j] (1 << 31 when z_j holds
+// that bit: the sign bit of the double +-1.0).
+struct PauliFlip {
+    uint32_t w[kFlipBits][kPauliTerms];
+    uint64_t bit[kFlipBits];
     uint64_t z[kPauliTerms];
 };
 
-// One pass for the terms j < nt that share x. The grid stride is a power of
-// two, so an index splits into disjoint bit sets: s = s_thread | s_iter(k),
-// s_thread from the work-item's index and s_iter from the loop counter k
-// alone (insert_run maps disjoint bit sets to disjoint bit sets). Hence
-// sigma_j(s) = sigma_j(s_thread) sigma_j(s_iter): the second factor is the
-// same for the whole wave and is formed on the scalar unit, the vector unit
-// adds v (+-) per term with FP64 FMAs (one for x = 0, where every term takes
-// the real part, two otherwise), and the first factor is applied once per
-// work-item at the end.
-template <typename T, int G>
-__global__ __launch_bounds__(kExThreads) void k_pauli(const cv<T>* __restrict__ psi, uint64_t npairs, uint64_t x,
-                                                      uint32_t hbit, uint32_t nt, PauliZ Z, uint32_t odd_mask,
-                                                      double* __restrict__ partial) {
-    __shared__ double lds[4];
-    double acc[G];
-#pragma unroll
-    for (int j = 0; j < G; ++j) acc[j] = 0;
-    const uint64_t stride = uint64_t(gridDim.x) * kExThreads;
-    const uint64_t t = uint64_t(blockIdx.x) * kExThreads + threadIdx.x;
-    const uint64_t st = x ? insert_run(t, hbit, 1) : t;
-    const bool live = t < npairs;  // false only when the state is smaller than one grid
-    for (uint64_t k0 = 0; k0 < npairs; k0 += stride) {
-        const uint64_t sk = x ? insert_run(k0, hbit, 1) : k0;  // the same for every work-item
-        double re = 0, im = 0;
-        if (live) {
-            const uint64_t s = st | sk;
-            if (x == 0) {
-                const cv<T> v = ldamp<T, true>(psi + s);
-                re = double(v.x) * double(v.x) + double(v.y) * double(v.y);
-            } else {
-                const cv<T> u = ldamp<T, true>(psi + s);
-                const cv<T> w = ldamp<T, true>(psi + (s ^ x));
-                re = double(u.x) * double(w.x) + double(u.y) * double(w.y);  // conj(u) w
-                im = double(u.x) * double(w.y) - double(u.y) * double(w.x);
-            }
-        }
-        if (x == 0) {  // every term even, im = 0: one FMA per term
-#pragma unroll
-            for (int j = 0; j < G; ++j) {
-                const double sg = (__popcll(sk & Z.z[j]) & 1) ? -1.0 : 1.0;
-                acc[j] = fma(re, sg, acc[j]);
-            }
-        } else {
-#pragma unroll
-            for (int j = 0; j < G; ++j) {
-                const double sg = (__popcll(sk & Z.z[j]) & 1) ? -1.0 : 1.0;
-                const double a = (odd_mask >> j & 1) ? 0.0 : sg, b = (odd_mask >> j & 1) ? sg : 0.0;
-                acc[j] = fma(re, a, fma(im, b, acc[j]));
-            }
-        }
+// One pass for the terms j < nt that share x (PAIR: x != 0, whose highest
+// bit is hbit). The grid stride is a power of two, so an index splits into
+// disjoint bit sets: s = s_thread | s_iter, s_thread from the work-item's
+// index and s_iter from the iteration alone (insert_run maps disjoint bit
+// sets to disjoint bit sets). Hence sigma_j(s) = sigma_j(s_thread)
+// sigma_j(s_iter): the second factor is the same for the whole wave, kept as
+// the high word of a double +-1.0 on the scalar unit and updated with one XOR
+// per term and iteration along the Gray walk; the vector unit adds v (+-) per
+// term with FP64 FMAs (x = 0: |psi_s|^2; x != 0: conj(psi_s) psi_{s^x}, real
+// and imaginary parts, of which the host keeps the one its term needs); the
+// first factor is applied once per work-item at the end.
+// Product term of one iteration: |psi_s|^2 (x = 0), or conj(psi_s) psi_{s^x}
+// as (re, im).
+template <typename T, bool PAIR>
+__device__ __forceinline__ void pauli_product(const cv<T>* __restrict__ psi, uint64_t s, uint64_t x, double& re,
+                                              double& im) {
+    if constexpr (PAIR) {
+        const cv<T> u = ldamp<T, true>(psi + s);
+        const cv<T> w = ldamp<T, true>(psi + (s ^ x));
+        re = double(u.x) * double(w.x) + double(u.y) * double(w.y);
+        im = double(u.x) * double(w.y) - double(u.y) * double(w.x);
+    } else {
+        const cv<T> v = ldamp<T, true>(psi + s);
+        re = double(v.x) * double(v.x) + double(v.y) * double(v.y);
+        im = 0;
     }
-#pragma unroll
-    for (int j = 0; j < G; ++j)
-        if (__popcll(st & Z.z[j]) & 1) acc[j] = -acc[j];
+}
+
+template <typename T, int G, bool PAIR>
+__global__ __launch_bounds__(kExThreads) void k_pauli(const cv<T>* __restrict__ psi, uint64_t npairs, uint64_t x,
+                                                      uint32_t hbit, uint32_t nt, uint32_t niter, uint32_t odd_mask,
+                                                      PauliFlip F, double* __restrict__ partial) {
+    // iterations per loop trip, their loads issued together: 4 single loads
+    // (x = 0) or 2 pairs
+    constexpr uint32_t U = PAIR ? 2 : 4;
+    __shared__ double lds[4][G];
+    double ar[G], ai[PAIR ? G : 1];
 #pragma unroll
     for (int j = 0; j < G; ++j) {
-        if (uint32_t(j) >= nt) break;
-        const double tot = block_sum256(acc[j], lds);
-        if (threadIdx.x == 0) partial[uint64_t(blockIdx.x) * nt + j] = tot;
+        ar[j] = 0;
+        if constexpr (PAIR) ai[j] = 0;
     }
+    uint64_t sg[G];  // the per-iteration signs (+-1.0) as bit patterns
+#pragma unroll
+    for (int j = 0; j < G; ++j) sg[j] = 0x3FF0000000000000ull;
+    const uint64_t t = uint64_t(blockIdx.x) * kExThreads + threadIdx.x;
+    // a work-item beyond a state smaller than one grid reads index 0 and
+    // adds nothing
+    const bool live = t < npairs;
+    const double wt = live ? 1.0 : 0.0;
+    const uint64_t st = live ? (PAIR ? insert_run(t, hbit, 1) : t) : 0;
+    auto acc = [&](double re, double im, int j) {
+        const double sgn = __longlong_as_double(static_cast<long long>(sg[j]));
+        ar[j] = fma(re, sgn, ar[j]);
+        if constexpr (PAIR) ai[j] = fma(im, sgn, ai[j]);
+    };
+    uint64_t sk = 0;
+    if (niter >= U) {
+        for (uint32_t k = 0; k < niter; k += U) {
+            if (k) {  // Gray step k flips iteration bit ctz(k) >= log2(U)
+                const uint32_t b = __builtin_ctz(k);
+                sk ^= F.bit[b];
+#pragma unroll
+                for (int j = 0; j < G; ++j) sg[j] ^= uint64_t(F.w[b][j]) << 32;
+            }
+            // steps k+1 .. k+U-1 flip bits 0 / 1 0 1 / 0
+            uint64_t s[U];
+            s[0] = st | sk;
+            s[1] = s[0] ^ F.bit[0];
+            if constexpr (U == 4) {
+                s[2] = s[1] ^ F.bit[1];
+                s[3] = s[2] ^ F.bit[0];
+            }
+            double re[U], im[U];
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u) pauli_product<T, PAIR>(psi, s[u], x, re[u], im[u]);
+#pragma unroll
+            for (int j = 0; j < G; ++j) {
+                acc(re[0], im[0], j);
+                sg[j] ^= uint64_t(F.w[0][j]) << 32;
+                acc(re[1], im[1], j);
+                if constexpr (U == 4) {
+                    sg[j] ^= uint64_t(F.w[1][j]) << 32;
+                    acc(re[2], im[2], j);
+                    sg[j] ^= uint64_t(F.w[0][j]) << 32;
+                    acc(re[3], im[3], j);
+                }
+            }
+            // sk and sg of the trip's last step, k + U - 1 (from step k:
+            // bit 1 for U = 4, the two flips of bit 0 cancelling; bit 0 for
+            // U = 2)
+            sk ^= U == 4 ? F.bit[1] : F.bit[0];
+        }
+    } else {
+        for (uint32_t k = 0; k < niter; ++k) {
+            if (k) {
+                const uint32_t b = __builtin_ctz(k);
+                sk ^= F.bit[b];
+#pragma unroll
+                for (int j = 0; j < G; ++j) sg[j] ^= uint64_t(F.w[b][j]) << 32;
+            }
+            double re, im;
+            pauli_product<T, PAIR>(psi, st | sk, x, re, im);
+#pragma unroll
+            for (int j = 0; j < G; ++j) acc(re, im, j);
+        }
+    }
+    // per term: the part the host needs, the work-item's sign, the wave's sum
+    const uint32_t w = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+        double v = ar[j];
+        if constexpr (PAIR)
+            if ((odd_mask >> j) & 1) v = ai[j];
+        v *= wt;
+        if (__popcll(st & F.z[j]) & 1) v = -v;
+        v = wave_sum(v);
+        if ((threadIdx.x & 63) == 0) lds[w][j] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < nt)
+        partial[uint64_t(blockIdx.x) * nt + threadIdx.x] =
+            ((lds[0][threadIdx.x] + lds[1][threadIdx.x]) + lds[2][threadIdx.x]) + lds[3][threadIdx.x];
 }
 
 // sum over blocks of the per-block partials of each term, fixed order
@@ -241,21 +312,43 @@ static int pauli_t(qcs_register* r, uint32_t nterms, const uint64_t* xmask, cons
         for (size_t t0 = 0; t0 < terms.size(); t0 += kPauliTerms) {
             const uint32_t nt = uint32_t(std::min<size_t>(kPauliTerms, terms.size() - t0));
             uint32_t odd = 0;
-            PauliZ Z{};
-            for (uint32_t j = 0; j < nt; ++j) {
-                const uint32_t ny = uint32_t(__builtin_popcountll(x & zmask[terms[t0 + j]]));
-                Z.z[j] = zmask[terms[t0 + j]];
-                if (ny & 1) odd |= 1u << j;
-            }
             // a power-of-two grid (the sign split needs a power-of-two stride)
             uint64_t nb = 1;
             while (nb < uint64_t(kPauliBlocks) && nb * kExThreads * 8 < npairs) nb <<= 1;
-            if (nt <= 8)
-                hipLaunchKernelGGL((k_pauli<T, 8>), dim3(unsigned(nb)), dim3(kExThreads), 0, r->stream,
-                                   static_cast<const cv<T>*>(r->psi), npairs, x, hbit, nt, Z, odd, r->red);
-            else
-                hipLaunchKernelGGL((k_pauli<T, kPauliTerms>), dim3(unsigned(nb)), dim3(kExThreads), 0, r->stream,
-                                   static_cast<const cv<T>*>(r->psi), npairs, x, hbit, nt, Z, odd, r->red);
+            const uint64_t stride = nb * kExThreads;
+            const uint32_t lstride = uint32_t(__builtin_ctzll(stride));
+            const uint64_t niter = npairs > stride ? npairs / stride : 1;
+            const uint32_t ibits = uint32_t(__builtin_ctzll(niter));
+            if (ibits > kFlipBits) return set_error(QCS_EINVAL, "state too large for the Pauli pass");
+            PauliFlip F{};
+            for (uint32_t j = 0; j < nt; ++j) {
+                const uint64_t z = zmask[terms[t0 + j]];
+                F.z[j] = z;
+                if (__builtin_popcountll(x & z) & 1) odd |= 1u << j;
+            }
+            for (uint32_t b = 0; b < ibits; ++b) {
+                const uint32_t kb = lstride + b, pos = (x && kb >= hbit) ? kb + 1 : kb;
+                F.bit[b] = 1ull << pos;
+                for (uint32_t j = 0; j < nt; ++j) F.w[b][j] = uint32_t((F.z[j] >> pos) & 1) << 31;
+            }
+            const cv<T>* psi = static_cast<const cv<T>*>(r->psi);
+            const dim3 grid{unsigned(nb)}, block{unsigned(kExThreads)};
+            const uint32_t ni = uint32_t(niter);
+            if (x == 0) {
+                if (nt <= 8)
+                    hipLaunchKernelGGL((k_pauli<T, 8, false>), grid, block, 0, r->stream, psi, npairs, x, hbit, nt, ni,
+                                       odd, F, r->red);
+                else
+                    hipLaunchKernelGGL((k_pauli<T, kPauliTerms, false>), grid, block, 0, r->stream, psi, npairs, x,
+                                       hbit, nt, ni, odd, F, r->red);
+            } else {
+                if (nt <= 8)
+                    hipLaunchKernelGGL((k_pauli<T, 8, true>), grid, block, 0, r->stream, psi, npairs, x, hbit, nt, ni,
+                                       odd, F, r->red);
+                else
+                    hipLaunchKernelGGL((k_pauli<T, kPauliTerms, true>), grid, block, 0, r->stream, psi, npairs, x,
+                                       hbit, nt, ni, odd, F, r->red);
+            }
             if (const int e = check_launch(r, "k_pauli")) return e;
             hipLaunchKernelGGL(k_pauli_final, dim3(1), dim3(kExThreads), 0, r->stream, r->red, uint32_t(nb), nt, res);
             if (const int e = check_launch(r, "k_pauli_final")) return e;

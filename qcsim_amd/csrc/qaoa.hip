@@ -68,10 +68,10 @@ constexpr uint32_t kTileRun = 8;    // tiles per tile unit (one instance)
 struct IsingLds {
     double h[kMaxSites];
     double J[kMaxEdges];
-    double aJ[2 * kMaxEdges];
     uint16_t aoff[kMaxSites + 1];
     uint8_t ei[kMaxEdges], ej[kMaxEdges];
     uint8_t aj[2 * kMaxEdges];
+    uint8_t ae[2 * kMaxEdges];  // the joining edge (index into J) of each neighbour
 };
 
 struct IsingArgsDev {
@@ -81,7 +81,7 @@ struct IsingArgsDev {
     const uint8_t* ej;
     const uint16_t* aoff;
     const uint8_t* aj;
-    const double* aJ;
+    const uint8_t* ae;
 };
 
 __device__ void load_params(IsingLds& P, const QInst& I, const IsingArgsDev& g) {
@@ -94,10 +94,13 @@ __device__ void load_params(IsingLds& P, const QInst& I, const IsingArgsDev& g) 
     }
     for (uint32_t k = threadIdx.x; k < 2 * I.ne; k += 256) {
         P.aj[k] = g.aj[I.boff + k];
-        P.aJ[k] = g.aJ[I.boff + k];
+        P.ae[k] = g.ae[I.boff + k];
     }
     __syncthreads();
 }
+
+// coupling of adjacency entry a
+__device__ __forceinline__ double adj_J(const IsingLds& P, uint32_t a) { return P.J[P.ae[a]]; }
 
 // hz = sum_i h_i z_i and jzz = sum_edges J z_i z_j of basis state s.
 __device__ __forceinline__ void ising_full(uint64_t s, uint32_t n, uint32_t ne, const IsingLds& P, double& hz,
@@ -111,7 +114,7 @@ __device__ __forceinline__ void ising_full(uint64_t s, uint32_t n, uint32_t ne, 
 // Update hz, jzz for flipping spin q of s (s is the state before the flip).
 __device__ __forceinline__ void ising_flip(uint64_t s, uint32_t q, const IsingLds& P, double& hz, double& jzz) {
     double f = 0;
-    for (uint32_t a = P.aoff[q]; a < P.aoff[q + 1]; ++a) f += ((s >> P.aj[a]) & 1) ? -P.aJ[a] : P.aJ[a];
+    for (uint32_t a = P.aoff[q]; a < P.aoff[q + 1]; ++a) f += ((s >> P.aj[a]) & 1) ? -adj_J(P, a) : adj_J(P, a);
     if ((s >> q) & 1) {  // z_q = -1
         hz += 2 * P.h[q];
         jzz += 2 * f;
@@ -185,11 +188,19 @@ __device__ void lds_rx(float2* t, uint32_t tb, float c, float s) {
     }
 }
 
-// LDS position of tile element i for tiles of >= 256 amplitudes: bits 0..4
-// XOR bits 5..9, which keeps every access pattern of tile_layer at the
-// minimum of two LDS cycles per wave.
-__device__ __forceinline__ uint32_t swz(uint32_t i) { return i ^ ((i >> 5) & 31u); }
+// LDS position of tile element i for tiles of >= 256 amplitudes: one pad
+// element after every 16, so that 16 lanes at a stride of 16 elements (the
+// round on qubits 0..3) fall in distinct banks. The map is additive over
+// index parts with disjoint bits when the lower part stays below 16 or the
+// upper part has bits 0..3 clear: an access base + (k << lo) for an orbit
+// base with bits lo..lo+3 clear is one base address per lane plus the
+// compile-time offset pad(k << lo), an immediate of the LDS instruction.
+__device__ __forceinline__ uint32_t swz(uint32_t i) { return i + (i >> 4); }
 __device__ __forceinline__ uint32_t tix(uint32_t i, uint32_t tb) { return tb >= 8 ? swz(i) : i; }
+// LDS elements of a tile of 2^tb amplitudes
+__host__ __device__ constexpr uint32_t tile_lds_elems(uint32_t tb) {
+    return (1u << tb) + (tb >= 8 ? (1u << (tb - 4)) : 0u);
+}
 
 template <int G>
 __device__ __forceinline__ void rx_reg(float2* x, float c, float s) {
@@ -258,12 +269,13 @@ __device__ void rx_round(float2* t, uint32_t tid, uint32_t tb, uint32_t lo, cons
     constexpr int D = 1 << G;
     for (uint32_t o = tid; o < (1u << (tb - G)); o += 256) {
         const uint32_t lb = uint32_t(insert_run(o, lo, G));
+        float2* const tp = t + swz(lb);  // lb has bits lo..lo+G-1 clear: swz is additive
         float2 y[D];
 #pragma unroll
-        for (int k = 0; k < D; ++k) y[k] = t[swz(lb + (uint32_t(k) << lo))];
+        for (int k = 0; k < D; ++k) y[k] = tp[swz(uint32_t(k) << lo)];
         rxs_reg<G>(y, rf);
 #pragma unroll
-        for (int k = 0; k < D; ++k) t[swz(lb + (uint32_t(k) << lo))] = y[k];
+        for (int k = 0; k < D; ++k) tp[swz(uint32_t(k) << lo)] = y[k];
     }
     __syncthreads();
 }
@@ -279,13 +291,13 @@ __device__ __forceinline__ uint32_t wave_orbit_base(uint32_t tid, uint32_t lo) {
 }
 
 __device__ __forceinline__ void rx_wave(float2* t, uint32_t tid, uint32_t lo, const RxForm& rf) {
-    const uint32_t lb = wave_orbit_base(tid, lo);
+    float2* const tp = t + swz(wave_orbit_base(tid, lo));  // base bits lo..lo+3 clear
     float2 y[16];
 #pragma unroll
-    for (int k = 0; k < 16; ++k) y[k] = t[swz(lb | (uint32_t(k) << lo))];
+    for (int k = 0; k < 16; ++k) y[k] = tp[swz(uint32_t(k) << lo)];
     rxs_reg<4>(y, rf);
 #pragma unroll
-    for (int k = 0; k < 16; ++k) t[swz(lb | (uint32_t(k) << lo))] = y[k];
+    for (int k = 0; k < 16; ++k) tp[swz(uint32_t(k) << lo)] = y[k];
 }
 
 // Orders this wave's LDS writes before its later LDS reads (the data a lane
@@ -299,10 +311,10 @@ __device__ __forceinline__ void lds_wave_sync() { asm volatile("s_waitcnt lgkmcn
 // gamma, q[m] = exp(-i gamma Q(m)) with Q(m) = sum of J z z over the edges
 // inside bits 8..tb-1 when those bits hold m.
 struct TileCostLds {
-    double F[12];
-    double K;
+    double F[kTileRun][12];
+    double K[kTileRun];
     float2 q[16];
-    float2 ef[4];
+    float2 ef[kTileRun][4];
 };
 
 // Thread t's share of the cost phase that depends on the instance and gamma
@@ -324,6 +336,32 @@ __device__ __forceinline__ float2 expmi(double th) {
 
 __device__ __forceinline__ double zf(uint64_t v, uint32_t b) { return ((v >> b) & 1) ? -1.0 : 1.0; }
 
+// The per-tile terms of the cost phase for the tiles tile0 .. tile0+ntile-1
+// (ntile <= kTileRun) of 2^(8+M) amplitudes, all at once: F[j][i] and
+// ef[j][i-8] by thread 32 j + i (i < tb), K[j] by thread 32 j + 31. The
+// barrier that ends thread_terms orders them before their use.
+template <int M>
+__device__ void tile_terms(TileCostLds& TC, double gamma, const QInst& I, const IsingLds& P, uint64_t tile0,
+                           uint32_t ntile) {
+    constexpr uint32_t tb = 8 + M;
+    const uint32_t j = threadIdx.x >> 5, i = threadIdx.x & 31;
+    if (j >= ntile) return;
+    const uint64_t base = (tile0 + j) << tb;
+    if (i < tb) {
+        double f = 0.5 * P.h[i];
+        for (uint32_t a = P.aoff[i]; a < P.aoff[i + 1]; ++a)
+            if (P.aj[a] >= tb) f += adj_J(P, a) * zf(base, P.aj[a]);
+        TC.F[j][i] = f;
+        if (i >= 8) TC.ef[j][i - 8] = expmi(-2.0 * f * gamma);
+    } else if (i == 31) {
+        double k = 0;
+        for (uint32_t s = tb; s < I.n; ++s) k += 0.5 * P.h[s] * zf(base, s);
+        for (uint32_t e = 0; e < I.ne; ++e)
+            if (P.ei[e] >= tb && P.ej[e] >= tb) k += P.J[e] * zf(base, P.ei[e]) * zf(base, P.ej[e]);
+        TC.K[j] = k;
+    }
+}
+
 // Fills W for this thread and TC.q, for tiles of 2^(8+M) amplitudes; ends
 // with a barrier.
 template <int M>
@@ -342,14 +380,14 @@ __device__ void thread_terms(ThreadTerms& W, double gamma, const QInst& I, const
     for (uint32_t i = 0; i < 8; ++i) {
         const double zi = zf(tid, i);
         for (uint32_t a = P.aoff[i]; a < P.aoff[i + 1]; ++a)
-            if (P.aj[a] < i) c += P.aJ[a] * zi * zf(tid, P.aj[a]);
+            if (P.aj[a] < i) c += adj_J(P, a) * zi * zf(tid, P.aj[a]);
     }
 #pragma unroll
     for (int k = 0; k < M; ++k) {
         const uint32_t i = 8 + k;
         double g = 0;
         for (uint32_t a = P.aoff[i]; a < P.aoff[i + 1]; ++a)
-            if (P.aj[a] < 8) g += P.aJ[a] * zf(tid, P.aj[a]);
+            if (P.aj[a] < 8) g += adj_J(P, a) * zf(tid, P.aj[a]);
         c += g;
         W.eg[k] = expmi(-2.0 * gamma * g);
     }
@@ -384,41 +422,23 @@ __device__ void thread_terms(ThreadTerms& W, double gamma, const QInst& I, const
 // so that the compiler recomputes the LDS addresses per tile instead of
 // holding them all in registers across a run of tiles.
 template <int M>
-__device__ void tile_layer(float2* t, uint32_t tid, uint64_t base, double gamma, const RxForm& rf, const QInst& I,
+__device__ void tile_layer(float2* t, uint32_t tid, uint32_t jt, double gamma, const RxForm& rf, const QInst& I,
                            const IsingLds& P, TileCostLds& TC, const ThreadTerms& W, int in_mode,
                            const float2* __restrict__ in, float amp) {
     constexpr uint32_t tb = 8 + M;
-    // per-tile terms: F and ef by the first tb threads, K by the second wave
-    if (tid < tb) {
-        double f = 0.5 * P.h[tid];
-        for (uint32_t a = P.aoff[tid]; a < P.aoff[tid + 1]; ++a)
-            if (P.aj[a] >= tb) f += P.aJ[a] * zf(base, P.aj[a]);
-        TC.F[tid] = f;
-        if (tid >= 8) TC.ef[tid - 8] = expmi(-2.0 * f * gamma);
-    } else if (tid >= 64 && tid < 128) {
-        const uint32_t l = tid - 64;
-        double k = 0;
-        for (uint32_t i = tb + l; i < I.n; i += 64) k += 0.5 * P.h[i] * zf(base, i);
-        for (uint32_t e = l; e < I.ne; e += 64)
-            if (P.ei[e] >= tb && P.ej[e] >= tb) k += P.J[e] * zf(base, P.ei[e]) * zf(base, P.ej[e]);
+    // per-thread terms, from the tile's terms (tile_terms, slot jt)
+    double A = TC.K[jt] + W.c0;
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) k += __shfl_xor(k, o, 64);
-        if (l == 0) TC.K = k;
-    }
-    __syncthreads();
-    // per-thread terms
-    double A = TC.K + W.c0;
-#pragma unroll
-    for (uint32_t i = 0; i < 8; ++i) A += zf(tid, i) * TC.F[i];
+    for (uint32_t i = 0; i < 8; ++i) A += zf(tid, i) * TC.F[jt][i];
     float2 e[M > 0 ? M : 1];
 #pragma unroll
     for (int k = 0; k < M; ++k) {
-        A += TC.F[8 + k];
+        A += TC.F[jt][8 + k];
         // exp(2 i gamma F'_k) = ef eg, multiplied and rescaled to modulus 1
         // (to O(delta^2) for |.|^2 = 1 + delta) in FP64, then rounded, so
         // that the walk's alternating products by it and its conjugate cancel
         // to FP32 rounding
-        const float2 a = TC.ef[k], b = W.eg[k];
+        const float2 a = TC.ef[jt][k], b = W.eg[k];
         const double x = double(a.x) * b.x - double(a.y) * b.y, y = double(a.x) * b.y + double(a.y) * b.x;
         const double r = 1.5 - 0.5 * (x * x + y * y);
         e[k] = make_float2(float(x * r), float(y * r));
@@ -441,13 +461,17 @@ __device__ void tile_layer(float2* t, uint32_t tid, uint64_t base, double gamma,
         const float wr = pr * qm.x - pi * qm.y, wi = pr * qm.y + pi * qm.x;
         const uint32_t i = tid + (gm << 8);
         const float2 v = in_mode == IN_GLOBAL ? ldamp<float, true>(in + i)
-                                              : (in_mode == IN_LDS ? t[swz(i)] : make_float2(amp, 0.f));
+                                              : (in_mode == IN_LDS ? t[swz(tid) + swz(gm << 8)] : make_float2(amp, 0.f));
         y[gm] = make_float2(wr * v.x - wi * v.y, wr * v.y + wi * v.x);
     }
     // y is the orbit of qubits 8..tb-1: their Rx before the LDS store
     if constexpr (M > 0) rxs_reg<M>(y, rf);
+    // every wave has read what it needs of the tile in LDS (the previous
+    // tile's store, or this layer's input) before it is overwritten; the
+    // global loads above are already in flight or done
+    __syncthreads();
 #pragma unroll
-    for (uint32_t m = 0; m < (1u << M); ++m) t[swz(tid + (m << 8))] = y[m];
+    for (uint32_t m = 0; m < (1u << M); ++m) t[swz(tid) + swz(m << 8)] = y[m];
     __syncthreads();
     if constexpr (M == 4) {
         // wave-local rounds; the caller orders the result before other waves
@@ -470,25 +494,25 @@ __device__ void tile_run(float2* t, float2* psi, uint64_t tile0, uint32_t ntile,
     constexpr uint32_t tb = 8 + M, C = 1u << tb;
     const RxForm rf = rx_form(c, s, tb);
     ThreadTerms W;
+    tile_terms<M>(TC, gamma, I, P, tile0, ntile);
     thread_terms<M>(W, gamma, I, P, TC);
     for (uint32_t j = 0; j < ntile; ++j) {
         const uint64_t base = (tile0 + j) << tb;
         uint32_t tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
         __builtin_assume(tid < 256u);
-        tile_layer<M>(t, tid, base, gamma, rf, I, P, TC, W, first ? IN_PLUS : IN_GLOBAL, psi + base, amp);
+        tile_layer<M>(t, tid, j, gamma, rf, I, P, TC, W, first ? IN_PLUS : IN_GLOBAL, psi + base, amp);
         if constexpr (M == 4) {
             // each wave stores the 1024 elements it rotated last
 #pragma unroll
             for (uint32_t r = 0; r < 16; ++r) {
-                const uint32_t i = ((tid >> 6) << 10) | (r << 6) | (tid & 63);
-                stamp<float, true>(psi + base + i, t[swz(i)]);
+                const uint32_t a = ((tid >> 6) << 10) | (tid & 63);
+                stamp<float, true>(psi + base + (a | (r << 6)), t[swz(a) + swz(r << 6)]);
             }
         } else {
 #pragma unroll
             for (uint32_t r = 0; r < C / 256; ++r) {
-                const uint32_t i = tid + 256 * r;
-                stamp<float, true>(psi + base + i, t[swz(i)]);
+                stamp<float, true>(psi + base + tid + 256 * r, t[swz(tid) + swz(256 * r)]);
             }
         }
     }
@@ -503,6 +527,7 @@ __device__ void small_run(float2* t, const double* an, uint32_t p, const QInst& 
         double sb, cb;
         sincos(an[2 * l + 1], &sb, &cb);
         ThreadTerms W;
+        tile_terms<M>(TC, an[2 * l], I, P, 0, 1);
         thread_terms<M>(W, an[2 * l], I, P, TC);
         uint32_t tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
@@ -689,7 +714,7 @@ __global__ __launch_bounds__(256) void k_qaoa_mix(float2* __restrict__ arena, co
                     double f = P.h[i];
                     for (uint32_t a = P.aoff[i]; a < P.aoff[i + 1]; ++a) {
                         const uint32_t j = P.aj[a];
-                        if (j < U.qlo || j >= U.qlo + g) f += ((base >> j) & 1) ? -P.aJ[a] : P.aJ[a];
+                        if (j < U.qlo || j >= U.qlo + g) f += ((base >> j) & 1) ? -adj_J(P, a) : adj_J(P, a);
                     }
                     g2[k] = 2.0 * f;
                 }
@@ -752,7 +777,8 @@ struct qcs_qaoa {
     QInst* d_inst = nullptr;
     uint8_t *d_ei = nullptr, *d_ej = nullptr, *d_aj = nullptr;
     uint16_t* d_aoff = nullptr;
-    double *d_J = nullptr, *d_h = nullptr, *d_aJ = nullptr;
+    double *d_J = nullptr, *d_h = nullptr;
+    uint8_t* d_ae = nullptr;
     double *d_ang = nullptr, *d_energy = nullptr, *d_partial = nullptr;
     uint32_t* d_large = nullptr;
     uint32_t nlarge = 0;
@@ -775,7 +801,7 @@ struct qcs_qaoa {
     float last_ms = 0;
     std::vector<QInst> inst_h;
 
-    IsingArgsDev args() const { return IsingArgsDev{d_h, d_J, d_ei, d_ej, d_aoff, d_aj, d_aJ}; }
+    IsingArgsDev args() const { return IsingArgsDev{d_h, d_J, d_ei, d_ej, d_aoff, d_aj, d_ae}; }
 };
 
 static void free_plan(qcs_qaoa* q) {
@@ -808,7 +834,7 @@ static int launch_sequence(qcs_qaoa* q) {
                 for (uint32_t i = 0; i < q->ninst; ++i)
                     if (q->nq[i] <= q->tmax) tb = std::max(tb, q->nq[i]);
             }
-            const size_t shm = sizeof(float2) << tb;
+            const size_t shm = sizeof(float2) * tile_lds_elems(tb);
             if (L.kind == qcs_qaoa::L_SMALL)
                 hipLaunchKernelGGL((k_qaoa_tile<true>), dim3(grid), dim3(256), shm, q->stream, q->arena, q->d_inst,
                                    L.units, L.nunits, tk, q->d_ang, q->p, ga, q->d_energy);
@@ -955,7 +981,7 @@ int qcs_qaoa_destroy(qcs_qaoa* q) {
     (void)hipFree(q->d_h);
     (void)hipFree(q->d_aoff);
     (void)hipFree(q->d_aj);
-    (void)hipFree(q->d_aJ);
+    (void)hipFree(q->d_ae);
     (void)hipFree(q->d_ang);
     (void)hipFree(q->d_energy);
     if (q->ev0) (void)hipEventDestroy(q->ev0);
@@ -1005,10 +1031,10 @@ int qcs_qaoa_create(uint32_t ninst, const uint32_t* nqubits, const uint32_t* ned
         e8j[k] = uint8_t(ej[k]);
     }
     // adjacency lists (CSR per instance): neighbours of each site with the
-    // coupling of the joining edge
+    // instance's index of the joining edge
     std::vector<uint16_t> aoff(ho + ninst + 1, 0);
     std::vector<uint8_t> aj(2 * eo + 1, 0);
-    std::vector<double> aJ(2 * eo + 1, 0.0);
+    std::vector<uint8_t> ae(2 * eo + 1, 0);
     uint32_t ao = 0;
     for (uint32_t i = 0; i < ninst; ++i) {
         QInst& I = q->inst_h[i];
@@ -1031,9 +1057,9 @@ int qcs_qaoa_create(uint32_t ninst, const uint32_t* nqubits, const uint32_t* ned
             const uint32_t a = ei[e0 + k], b = ej[e0 + k];
             const uint32_t pa = I.boff + aoff[ao + a] + fill[a]++, pb = I.boff + aoff[ao + b] + fill[b]++;
             aj[pa] = uint8_t(b);
-            aJ[pa] = J[e0 + k];
+            ae[pa] = uint8_t(k);
             aj[pb] = uint8_t(a);
-            aJ[pb] = J[e0 + k];
+            ae[pb] = uint8_t(k);
         }
         ao += n + 1;
     }
@@ -1045,7 +1071,7 @@ int qcs_qaoa_create(uint32_t ninst, const uint32_t* nqubits, const uint32_t* ned
         !al(reinterpret_cast<void**>(&q->d_h), sizeof(double) * (ho + 1)) ||
         !al(reinterpret_cast<void**>(&q->d_aoff), sizeof(uint16_t) * aoff.size()) ||
         !al(reinterpret_cast<void**>(&q->d_aj), aj.size()) ||
-        !al(reinterpret_cast<void**>(&q->d_aJ), sizeof(double) * aJ.size()) ||
+        !al(reinterpret_cast<void**>(&q->d_ae), ae.size()) ||
         !al(reinterpret_cast<void**>(&q->d_ang), sizeof(double) * 2 * p * ninst) ||
         !al(reinterpret_cast<void**>(&q->d_energy), sizeof(double) * ninst)) {
         (void)hipGetLastError();
@@ -1058,7 +1084,7 @@ int qcs_qaoa_create(uint32_t ninst, const uint32_t* nqubits, const uint32_t* ned
         hipMemcpy(q->d_h, h, sizeof(double) * ho, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(q->d_aoff, aoff.data(), sizeof(uint16_t) * aoff.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(q->d_aj, aj.data(), aj.size(), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(q->d_aJ, aJ.data(), sizeof(double) * aJ.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(q->d_ae, ae.data(), ae.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipStreamCreateWithFlags(&q->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&q->ev0) != hipSuccess || hipEventCreate(&q->ev1) != hipSuccess) {
         qcs_qaoa_destroy(q);

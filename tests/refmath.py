@@ -78,3 +78,35 @@ def dense_pauli(x, z, n):
         xb, zb = (x >> q) & 1, (z >> q) & 1
         m = np.kron(Y2 if xb and zb else X2 if xb else Z2 if zb else I2, m)
     return m
+
+
+def pauli_ev(psi, x, z):
+    """<psi|P|psi> of the Pauli string with masks (x, z) (as dense_pauli)
+    without the matrix: (P psi)_s = i^{|x & z|} (-1)^{popcount((s ^ x) & z)}
+    psi_{s ^ x}, the row form of the Kronecker product (Y = i X Z per
+    qubit), vectorised over all s."""
+    psi = np.asarray(psi, dtype=np.complex128)
+    s = np.arange(psi.shape[0], dtype=np.uint64)
+    t = s ^ np.uint64(x)
+    sign = 1.0 - 2.0 * (np.bitwise_count(t & np.uint64(z)) & 1)
+    ny = bin(x & z).count("1")
+    pp = (1j ** ny) * sign * psi[t]
+    return float(np.vdot(psi, pp).real)
+
+
+def pauli_evs(psi, xs, zs):
+    """pauli_ev over many terms, the products conj(psi_s) psi_{s^x} formed
+    once per distinct x mask."""
+    psi = np.asarray(psi, dtype=np.complex128)
+    s = np.arange(psi.shape[0], dtype=np.uint64)
+    out = np.zeros(len(xs))
+    for x in sorted(set(xs)):
+        t = s ^ np.uint64(x)
+        prod = np.conj(psi) * psi[t]
+        for k, (xk, z) in enumerate(zip(xs, zs)):
+            if xk != x:
+                continue
+            odd = (np.bitwise_count(t & np.uint64(z)) & 1).astype(bool)
+            tot = prod.sum() - 2.0 * prod[odd].sum()
+            out[k] = float(((1j ** bin(x & z).count("1")) * tot).real)
+    return out

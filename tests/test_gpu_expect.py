@@ -20,7 +20,7 @@ import pytest
 from qcsim_amd import QubitRegister
 from qcsim_amd import gates as G
 from qcsim_amd import pauli as P
-from refmath import dense_pauli, random_state
+from refmath import dense_pauli, pauli_evs, random_state
 
 pytestmark = pytest.mark.gpu
 
@@ -59,6 +59,33 @@ def test_pauli_strings_match_dense_operator(n, precision):
     assert abs(r2.pauli_expectation([0], [1])[0]) < 1e-15
     r.close()
     r2.close()
+
+
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+def test_pauli_strings_at_24_qubits(precision):
+    """At 24 qubits a pass loops over 16 (X != 0) or 32 (X = 0) grid strides
+    in Gray order, flipping the terms' signs from a per-bit table; 18 terms
+    per X mask take a full pass of 16 and one of 2. Reference: the row form
+    of each string on the stored state."""
+    n = 24
+    rng = np.random.default_rng(24)
+    psi = random_state(rng, n)
+    r = QubitRegister(n, precision=precision)
+    r.setRegisterStorageFastNoNormalize(psi)
+    state = r.getRegisterStorage()
+    xs, zs = [], []
+    for x in (0, (1 << 23) | 5, 1):
+        for _ in range(18):
+            xs.append(x)
+            zs.append(int(rng.integers(0, 1 << n)))
+    zs[0] = 0  # <I> = 1
+    got = r.pauli_expectation(xs, zs)
+    want = pauli_evs(state, xs, zs)
+    err = float(np.max(np.abs(got - want)))
+    print(f"24 qubits {precision}: {len(xs)} Pauli terms, max error {err:.2e}")
+    assert abs(got[0] - float(np.vdot(state, state).real)) < 1e-12
+    assert err < TOL[precision]
+    r.close()
 
 
 def test_pauli_masks_beyond_register_rejected():

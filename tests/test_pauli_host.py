@@ -6,7 +6,7 @@ import numpy as np
 
 from qcsim_amd import gates as G
 from qcsim_amd import pauli as P
-from refmath import dense_pauli
+from refmath import dense_pauli, pauli_ev, pauli_evs, random_state
 
 
 def test_pauli_string_masks_default_to_z():
@@ -57,3 +57,20 @@ def test_dense_pauli_y_convention():
     z = dense_pauli(0, 1, 1)
     assert np.allclose(dense_pauli(1, 1, 1), 1j * x @ z)
     assert np.allclose(dense_pauli(1, 0, 2), np.kron(np.eye(2), x))
+
+
+def test_row_form_matches_the_dense_string():
+    """refmath.pauli_ev / pauli_evs (the row form, used as the reference at
+    sizes where the dense matrix does not fit) against the Kronecker
+    product."""
+    rng = np.random.default_rng(5)
+    n = 6
+    psi = random_state(rng, n)
+    xs, zs, want = [], [], []
+    for _ in range(100):
+        x, z = int(rng.integers(0, 1 << n)), int(rng.integers(0, 1 << n))
+        xs.append(x)
+        zs.append(z)
+        want.append(np.vdot(psi, dense_pauli(x, z, n) @ psi).real)
+        assert abs(pauli_ev(psi, x, z) - want[-1]) < 1e-14, (x, z)
+    assert np.max(np.abs(pauli_evs(psi, xs, zs) - np.array(want))) < 1e-14
