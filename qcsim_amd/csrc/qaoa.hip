@@ -61,6 +61,18 @@ struct QUnit {
 };
 
 constexpr uint32_t kMaxSites = 64, kMaxEdges = 256;
+
+// Amplitude i of an instance is stored at position i ^ arena_xor(i): bits
+// 4..9 of i XOR a 6-bit fold of its bits 18.. (two fields for up to 30
+// qubits). The 64 rows of a mixer orbit on qubits 18..23 (2 MiB apart) then
+// sit at different offsets of their 8 KiB blocks instead of one; an orbit on
+// qubits below 18 sees one constant XOR, a tile (bits < 12) is permuted
+// within itself (aligned runs of 64 stay runs of 64), and an instance of <= 18
+// qubits is stored in order. The map is linear over XOR and self-inverse (the
+// mask reads only bits >= 18 and writes only bits 4..9).
+__host__ __device__ __forceinline__ uint64_t arena_xor(uint64_t i) {
+    return (((i >> 18) ^ (i >> 24)) & 63u) << 4;
+}
 constexpr uint32_t kMixChunks = 4;  // 256-orbit chunks per mixer unit
 constexpr uint32_t kTileRun = 8;    // tiles per tile unit (one instance)
 
@@ -401,9 +413,10 @@ __device__ void thread_terms(ThreadTerms& W, double gamma, const QInst& I, const
 // Thread t takes elements t + 256 m (m < 2^M: the tile bits 8..tb-1) in Gray
 // order over m and applies the cost phase; these 2^M elements are the orbit
 // of qubits 8..tb-1, so their Rx follows in registers before the elements go
-// to the (swizzled) LDS tile; two rounds then apply Rx on qubits 4..7 and
-// 0..3 with orbits held in registers. The input is |+> (amplitude amp), `in`
-// (global) or the LDS tile; the result is left in the LDS tile.
+// to the (padded) LDS tile; two rounds then apply Rx on qubits 4..7 and
+// 0..3 with orbits held in registers. The input is |+> (amplitude amp), the
+// stored tile in global memory (element i at in[i ^ ci]) or the LDS tile;
+// the result is left in the LDS tile.
 //
 // The cost phase splits along the tile: with z_i = 1 - 2 b_i,
 //   C = K + A(t) + sum_{i in 8..tb-1} z_i F'_i(t) + Q(m),
@@ -424,7 +437,7 @@ __device__ void thread_terms(ThreadTerms& W, double gamma, const QInst& I, const
 template <int M>
 __device__ void tile_layer(float2* t, uint32_t tid, uint32_t jt, double gamma, const RxForm& rf, const QInst& I,
                            const IsingLds& P, TileCostLds& TC, const ThreadTerms& W, int in_mode,
-                           const float2* __restrict__ in, float amp) {
+                           const float2* __restrict__ in, uint32_t ci, float amp) {
     constexpr uint32_t tb = 8 + M;
     // per-thread terms, from the tile's terms (tile_terms, slot jt)
     double A = TC.K[jt] + W.c0;
@@ -460,7 +473,7 @@ __device__ void tile_layer(float2* t, uint32_t tid, uint32_t jt, double gamma, c
         const float pr = ph.x, pi = ph.y;
         const float wr = pr * qm.x - pi * qm.y, wi = pr * qm.y + pi * qm.x;
         const uint32_t i = tid + (gm << 8);
-        const float2 v = in_mode == IN_GLOBAL ? ldamp<float, true>(in + i)
+        const float2 v = in_mode == IN_GLOBAL ? ldamp<float, true>(in + (i ^ ci))
                                               : (in_mode == IN_LDS ? t[swz(tid) + swz(gm << 8)] : make_float2(amp, 0.f));
         y[gm] = make_float2(wr * v.x - wi * v.y, wr * v.y + wi * v.x);
     }
@@ -501,18 +514,25 @@ __device__ void tile_run(float2* t, float2* psi, uint64_t tile0, uint32_t ntile,
         uint32_t tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
         __builtin_assume(tid < 256u);
-        tile_layer<M>(t, tid, j, gamma, rf, I, P, TC, W, first ? IN_PLUS : IN_GLOBAL, psi + base, amp);
+        // stored positions: element i of the tile at base is io[i ^ ci]
+        // (io = psi + base for tiles of >= 1024 amplitudes; a smaller tile
+        // trades elements with the tile at base ^ (the mask's bits >= tb),
+        // which has the same mask)
+        const uint64_t cx = arena_xor(base);
+        float2* const io = psi + (base ^ (cx & ~uint64_t(C - 1)));
+        const uint32_t ci = uint32_t(cx) & (C - 1);
+        tile_layer<M>(t, tid, j, gamma, rf, I, P, TC, W, first ? IN_PLUS : IN_GLOBAL, io, ci, amp);
         if constexpr (M == 4) {
             // each wave stores the 1024 elements it rotated last
 #pragma unroll
             for (uint32_t r = 0; r < 16; ++r) {
                 const uint32_t a = ((tid >> 6) << 10) | (tid & 63);
-                stamp<float, true>(psi + base + (a | (r << 6)), t[swz(a) + swz(r << 6)]);
+                stamp<float, true>(io + ((a | (r << 6)) ^ ci), t[swz(a) + swz(r << 6)]);
             }
         } else {
 #pragma unroll
             for (uint32_t r = 0; r < C / 256; ++r) {
-                stamp<float, true>(psi + base + tid + 256 * r, t[swz(tid) + swz(256 * r)]);
+                stamp<float, true>(io + ((tid + 256 * r) ^ ci), t[swz(tid) + swz(256 * r)]);
             }
         }
     }
@@ -532,7 +552,7 @@ __device__ void small_run(float2* t, const double* an, uint32_t p, const QInst& 
         uint32_t tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
         __builtin_assume(tid < 256u);
-        tile_layer<M>(t, tid, 0, an[2 * l], rx_form(cb, sb, 8 + M), I, P, TC, W, l ? IN_LDS : IN_PLUS, nullptr, amp);
+        tile_layer<M>(t, tid, 0, an[2 * l], rx_form(cb, sb, 8 + M), I, P, TC, W, l ? IN_LDS : IN_PLUS, nullptr, 0, amp);
     }
     __syncthreads();  // the energy pass reads every wave's elements
 }
@@ -612,7 +632,7 @@ __global__ __launch_bounds__(256, 4) void k_qaoa_tile(float2* __restrict__ arena
                 for (uint32_t m = 0;; ++m) {
                     const float2 x = tile[tix(uint32_t(s), I.n)];
                     acc += (double(x.x) * x.x + double(x.y) * x.y) * energy_of(hz, jzz);
-                    psi[s] = x;
+                    psi[s] = x;  // n <= 12: stored in order
                     if (m + 1 == per) break;
                     const uint32_t q = 8 + __builtin_ctz(m + 1);
                     ising_flip(s, q, P, hz, jzz);
@@ -633,11 +653,13 @@ __global__ __launch_bounds__(256, 4) void k_qaoa_tile(float2* __restrict__ arena
                 for (uint32_t j = 0; j < U.ntile; ++j) {
                     const uint64_t base = (uint64_t(U.idx) + j) << tb;
                     for (uint32_t i = threadIdx.x; i < C; i += 256)
-                        tile[i] = U.layer ? ldamp<float, true>(psi + base + i) : make_float2(amp, 0.f);
+                        tile[i] = U.layer ? ldamp<float, true>(psi + ((base + i) ^ arena_xor(base)))
+                                          : make_float2(amp, 0.f);
                     __syncthreads();
                     lds_cost(tile, tb, base, an[2 * U.layer], I, P);
                     lds_rx(tile, tb, float(cb), float(sb));
-                    for (uint32_t i = threadIdx.x; i < C; i += 256) stamp<float, true>(psi + base + i, tile[i]);
+                    for (uint32_t i = threadIdx.x; i < C; i += 256)
+                        stamp<float, true>(psi + ((base + i) ^ arena_xor(base)), tile[i]);
                     __syncthreads();
                 }
             }
@@ -690,9 +712,16 @@ __global__ __launch_bounds__(256) void k_qaoa_mix(float2* __restrict__ arena, co
             const uint64_t o = (uint64_t(U.idx) + r) * 256 + threadIdx.x;
             if (o >= norb) break;
             const uint64_t base = insert_run(o, U.qlo, g);
+            // stored position of base | c << qlo: pb ^ row(c) (the two parts
+            // are disjoint and the mask is linear)
+            const uint64_t pb = base ^ arena_xor(base);
+            auto row = [&](int c) {
+                const uint64_t v = uint64_t(c) << U.qlo;
+                return v ^ arena_xor(v);
+            };
             float2 x[D];
 #pragma unroll
-            for (int c = 0; c < D; ++c) x[c] = ldamp<float, true>(psi + (base | (uint64_t(c) << U.qlo)));
+            for (int c = 0; c < D; ++c) x[c] = ldamp<float, true>(psi + (pb ^ row(c)));
 #pragma unroll
             for (int q = 0; q < g; ++q)
 #pragma unroll
@@ -733,10 +762,10 @@ __global__ __launch_bounds__(256) void k_qaoa_mix(float2* __restrict__ arena, co
             // compiler cannot see through, so that it does not keep all 2^g
             // 64-bit load addresses live across the rotations (for g = 6 that
             // alone would be 128 VGPRs).
-            uint64_t sbase = base;
+            uint64_t sbase = pb;
             asm volatile("" : "+v"(sbase));
 #pragma unroll
-            for (int c = 0; c < D; ++c) stamp<float, true>(psi + (sbase | (uint64_t(c) << U.qlo)), x[c]);
+            for (int c = 0; c < D; ++c) stamp<float, true>(psi + (sbase ^ row(c)), x[c]);
         }
         if (E) {
             const double tot = block_sum(acc, red);
@@ -1153,8 +1182,9 @@ int qcs_qaoa_get_state(qcs_qaoa* q, uint32_t inst, double* amps) {
         hipStreamSynchronize(q->stream) != hipSuccess)
         return set_error(QCS_EDEVICE, "state download failed");
     for (uint64_t i = 0; i < A; ++i) {
-        amps[2 * i] = tmp[i].x;
-        amps[2 * i + 1] = tmp[i].y;
+        const float2 v = tmp[i ^ arena_xor(i)];
+        amps[2 * i] = v.x;
+        amps[2 * i + 1] = v.y;
     }
     return QCS_OK;
 }
