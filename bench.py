@@ -600,11 +600,14 @@ def host_info():
     return info
 
 
-def cpu_baseline(n, targets, seed=7, reps=5, single_targets=(15,), single_reps=3):
+def cpu_baseline(n, targets, seed=7, reps=5, single_targets=(15,), single_reps=3, sse2_targets=(0, 15, 29),
+                 sse2_reps=3):
     """The oracle (the reference's loops restated: results buffer + swap,
     OpenMP) timed per gate: for each target one warm-up gate, then `reps`
-    timed gates; the figure is the median of the per-target medians. Also a
-    1-thread run on `single_targets` and the host's STREAM triad."""
+    timed gates; the figure is the median of the per-target medians (the
+    AVX2 build, the reference's Release flags). Also a 1-thread run on
+    `single_targets`, the same loops built with the reference's CMake flags
+    (-msse2) on `sse2_targets`, and the host's STREAM triad."""
     from oracle.oracle import OracleRegister, lib
     from qcsim_amd import gates as G
 
@@ -632,11 +635,25 @@ def cpu_baseline(n, targets, seed=7, reps=5, single_targets=(15,), single_reps=3
             ts.append(time.perf_counter() - t0)
         single[q] = float(np.median(ts))
     r.close()
+    # the same loops built with the reference's CMake flags (-msse2 -mpopcnt)
+    sse2 = {}
+    r = OracleRegister(n, isa="sse2")
+    for q in sse2_targets:
+        g = G.SingleQubitGate(G.haar_u2(rng))
+        r.apply_gate(g, q)
+        ts = []
+        for _ in range(sse2_reps):
+            t0 = time.perf_counter()
+            r.apply_gate(g, q)
+            ts.append(time.perf_counter() - t0)
+        sse2[q] = float(np.median(ts))
+    r.close()
     cores = int(lib().orc_max_threads())
     triad_all = float(lib().orc_stream_triad(1 << 27, 5, 0))
     triad_one = float(lib().orc_stream_triad(1 << 27, 5, 1))
     per_gate = float(np.median(list(per_target.values())))
     one = float(np.median(list(single.values())))
+    per_gate_sse2 = float(np.median(list(sse2.values())))
     gb = algorithmic_bytes_1q(n) / 1e9
     return {
         "value": round(1.0 / per_gate, 4),
@@ -652,6 +669,11 @@ def cpu_baseline(n, targets, seed=7, reps=5, single_targets=(15,), single_reps=3
                           "per_target_ms": {int(q): round(t * 1e3, 1) for q, t in single.items()}},
         "stream_triad_GBps": {"all_threads": round(triad_all, 1), "one_thread": round(triad_one, 1)},
         "flags": "g++ -O3 -fopenmp -mavx2 -ffp-contract=off",
+        "sse2": {"flags": "g++ -O3 -fopenmp -msse2 -mpopcnt -ffp-contract=off",
+                 "gates_per_s": round(1.0 / per_gate_sse2, 4),
+                 "sample": f"targets {list(sse2_targets)}: one warm-up and {sse2_reps} timed gates per target, "
+                 f"median of the per-target medians {per_gate_sse2 * 1e3:.1f} ms",
+                 "per_target_ms": {int(q): round(t * 1e3, 2) for q, t in sse2.items()}},
         "host": host_info(),
     }
 
@@ -726,7 +748,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(n, [0, 7, 15, 22, 29] if n >= 30 else list(range(0, n, max(1, n // 5))),
-                           single_targets=(min(15, n - 1),))
+                           single_targets=(min(15, n - 1),), sse2_targets=(0, n // 2, n - 1))
         progress("cpu baseline done")
         sampling["cpu_reference"] = cpu_config0(c0_seq, c0_u, c0_psi, c0_out)
         progress("cpu reference of the sampling workload done")

@@ -13,6 +13,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "libqcs_oracle.so")
+# the same source built with -msse2 -mpopcnt (the reference's CMake flags on
+# x86-64 Linux) instead of -mavx2; timed beside the AVX2 build
+LIB_SSE2 = os.path.join(HERE, "libqcs_oracle_sse2.so")
 
 _PD = C.POINTER(C.c_double)
 _PU32 = C.POINTER(C.c_uint32)
@@ -23,15 +26,17 @@ def build() -> None:
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB):
+def lib(isa: str = "avx2"):
+    """The oracle library: isa "avx2" (the default, used by the tests) or
+    "sse2"."""
+    if isa not in _libs:
+        path = {"avx2": LIB, "sse2": LIB_SSE2}[isa]
+        if not os.path.exists(path):
             build()
-        L = C.CDLL(LIB)
+        L = C.CDLL(path)
         sig = {
             "orc_create": (C.c_void_p, [C.c_uint32]),
             "orc_destroy": (None, [C.c_void_p]),
@@ -60,8 +65,8 @@ def lib():
             f = getattr(L, k)
             f.restype = r
             f.argtypes = a
-        _lib = L
-    return _lib
+        _libs[isa] = L
+    return _libs[isa]
 
 
 def _d(a):
@@ -75,17 +80,18 @@ def _mat(m):
 class OracleRegister:
     """CPU restatement of QC::QubitRegister (complex128, results buffer + swap)."""
 
-    def __init__(self, n: int, multithreading: bool = True):
+    def __init__(self, n: int, multithreading: bool = True, isa: str = "avx2"):
         self.n = n
         self.A = 1 << n
-        self._h = lib().orc_create(n)
+        self._L = lib(isa)
+        self._h = self._L.orc_create(n)
         if not self._h:
             raise MemoryError("oracle register allocation failed")
-        lib().orc_set_multithreading(self._h, int(multithreading))
+        self._L.orc_set_multithreading(self._h, int(multithreading))
 
     def close(self):
         if self._h:
-            lib().orc_destroy(self._h)
+            self._L.orc_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -97,72 +103,72 @@ class OracleRegister:
     def set_state(self, psi):
         v = np.ascontiguousarray(psi, dtype=np.complex128)
         assert v.shape[0] == self.A
-        lib().orc_set_state(self._h, _d(v.view(np.float64)))
+        self._L.orc_set_state(self._h, _d(v.view(np.float64)))
 
     def state(self) -> np.ndarray:
         out = np.empty(self.A, dtype=np.complex128)
-        lib().orc_get_state(self._h, _d(out.view(np.float64)))
+        self._L.orc_get_state(self._h, _d(out.view(np.float64)))
         return out
 
     def apply_gate(self, gate, q, c1=0, c2=0):
-        rc = lib().orc_apply_gate(self._h, gate.getQubitsNumber(), gate.traits, q, c1, c2,
+        rc = self._L.orc_apply_gate(self._h, gate.getQubitsNumber(), gate.traits, q, c1, c2,
                                   _d(_mat(gate.getRawOperatorMatrix())))
         if rc:
-            raise ValueError(lib().orc_last_error().decode())
+            raise ValueError(self._L.orc_last_error().decode())
 
     def apply_raw(self, k, traits, q, c1, c2, m):
-        rc = lib().orc_apply_gate(self._h, k, traits, q, c1, c2, _d(_mat(m)))
+        rc = self._L.orc_apply_gate(self._h, k, traits, q, c1, c2, _d(_mat(m)))
         if rc:
-            raise ValueError(lib().orc_last_error().decode())
+            raise ValueError(self._L.orc_last_error().decode())
 
     def apply_dense(self, qubits, m):
         q = (C.c_uint32 * len(qubits))(*qubits)
-        if lib().orc_apply_dense(self._h, len(qubits), q, _d(_mat(m))):
+        if self._L.orc_apply_dense(self._h, len(qubits), q, _d(_mat(m))):
             raise ValueError("bad dense block")
 
     def measure_all(self, u):
-        return int(lib().orc_measure_all(self._h, u))
+        return int(self._L.orc_measure_all(self._h, u))
 
     def measure(self, first, second, u):
-        return int(lib().orc_measure(self._h, first, second, u))
+        return int(self._L.orc_measure(self._h, first, second, u))
 
     def measure_no_collapse(self, u, first=None, second=None):
         full = first is None
-        return int(lib().orc_measure_no_collapse(self._h, int(full), first or 0,
+        return int(self._L.orc_measure_no_collapse(self._h, int(full), first or 0,
                                                  (second if second is not None else (first or 0)), u))
 
     def sample(self, u, first=None, second=None):
         u = np.ascontiguousarray(u, dtype=np.float64)
         out = np.empty(u.shape[0], dtype=np.uint64)
         full = first is None
-        lib().orc_sample(self._h, int(full), first or 0, second or 0, u.shape[0], _d(u),
+        self._L.orc_sample(self._h, int(full), first or 0, second or 0, u.shape[0], _d(u),
                          out.ctypes.data_as(_PU64))
         return out
 
     def qubit_probability(self, q):
-        return float(lib().orc_qubit_probability(self._h, q))
+        return float(self._L.orc_qubit_probability(self._h, q))
 
     def norm2(self):
-        return float(lib().orc_norm2(self._h))
+        return float(self._L.orc_norm2(self._h))
 
     def normalize(self):
-        lib().orc_normalize(self._h)
+        self._L.orc_normalize(self._h)
 
     def inner(self, other):
         out = np.zeros(2)
-        lib().orc_inner(self._h, other._h, _d(out))
+        self._L.orc_inner(self._h, other._h, _d(out))
         return complex(out[0], out[1])
 
     def qft(self, sq=0, eq=None, do_swap=True, inverse=False):
         eq = self.n - 1 if eq is None else eq
-        lib().orc_qft(self._h, sq, eq, int(do_swap), int(inverse))
+        self._L.orc_qft(self._h, sq, eq, int(do_swap), int(inverse))
 
     def ising_energy(self, edges, J, h):
         ei = np.ascontiguousarray([e[0] for e in edges], dtype=np.uint32)
         ej = np.ascontiguousarray([e[1] for e in edges], dtype=np.uint32)
         Jv = np.ascontiguousarray(J, dtype=np.float64)
         hv = np.ascontiguousarray(h, dtype=np.float64)
-        return float(lib().orc_ising_energy(self._h, len(ei), ei.ctypes.data_as(_PU32), ej.ctypes.data_as(_PU32),
+        return float(self._L.orc_ising_energy(self._h, len(ei), ei.ctypes.data_as(_PU32), ej.ctypes.data_as(_PU32),
                                             _d(Jv), _d(hv)))
 
     def qaoa_exec(self, edges, J, h, gamma, beta, better_mixing=False):
@@ -172,7 +178,7 @@ class OracleRegister:
         hv = np.ascontiguousarray(h, dtype=np.float64)
         g = np.ascontiguousarray(gamma, dtype=np.float64)
         b = np.ascontiguousarray(beta, dtype=np.float64)
-        lib().orc_qaoa_exec(self._h, len(ei), ei.ctypes.data_as(_PU32), ej.ctypes.data_as(_PU32), _d(Jv), _d(hv),
+        self._L.orc_qaoa_exec(self._h, len(ei), ei.ctypes.data_as(_PU32), ej.ctypes.data_as(_PU32), _d(Jv), _d(hv),
                             len(g), _d(g), _d(b), int(better_mixing))
 
 
