@@ -17,7 +17,8 @@
 //   n <= T: one SMALL unit runs every layer and the energy with the whole
 //           state in LDS (k_qaoa_tile<true>);
 //   else per layer
-//     tile launch (k_qaoa_tile<false>): 2^(n-T) units, each a 2^T tile in
+//     tile launch (k_qaoa_tile<false>): the 2^(n-T) tiles of 2^T
+//           amplitudes, in units of up to 8 consecutive tiles, each tile in
 //           LDS: (layer 0: |+> generated in place) cost diagonal, Rx on
 //           qubits 0..T-1;
 //     mixer launches (k_qaoa_mix<g, E>) j = 0 .. ceil((n-T)/G) - 1: Rx on
@@ -25,7 +26,8 @@
 //           2^g-amplitude orbit in registers; the last one of the last layer
 //           also accumulates the energy (E).
 // A launch pulls units from its own atomic ticket until none are left; the
-// launches of one evaluation are captured in a hipGraph.
+// launches of one evaluation are captured in a hipGraph. Instances of more
+// than 18 qubits are stored in a permuted order (arena_xor below).
 //
 // C(s) and E(s) are walked in Gray-code order over the bits a thread owns:
 // one full evaluation (all sites and edges) per thread, then per step the
