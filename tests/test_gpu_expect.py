@@ -62,6 +62,28 @@ def test_pauli_strings_match_dense_operator(n, precision):
 
 
 @pytest.mark.parametrize("precision", ["c128", "c64"])
+@pytest.mark.parametrize("n", [10, 11, 13])
+def test_pauli_strings_across_pass_shapes(n, precision):
+    """Sizes whose passes take 1, 2, 4 and 8 loop iterations per work-item
+    (with and without the 4- and 2-iteration trips), 24 terms per X mask
+    (a pass of 16 and one of 8). Reference: the row form on the stored
+    state."""
+    rng = np.random.default_rng(40 + n)
+    r = QubitRegister(n, precision=precision)
+    r.setRegisterStorageFastNoNormalize(random_state(rng, n))
+    state = r.getRegisterStorage()
+    xs, zs = [], []
+    for x in (0, 1, 1 << (n - 1), int(rng.integers(1, 1 << n))):
+        for _ in range(24):
+            xs.append(x)
+            zs.append(int(rng.integers(0, 1 << n)))
+    err = float(np.max(np.abs(r.pauli_expectation(xs, zs) - pauli_evs(state, xs, zs))))
+    print(f"n={n} {precision}: {len(xs)} Pauli terms, max error {err:.2e}")
+    assert err < TOL[precision]
+    r.close()
+
+
+@pytest.mark.parametrize("precision", ["c128", "c64"])
 def test_pauli_strings_at_24_qubits(precision):
     """At 24 qubits a pass loops over 16 (X != 0) or 32 (X = 0) grid strides
     in Gray order, flipping the terms' signs from a per-bit table; 18 terms
