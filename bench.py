@@ -330,7 +330,7 @@ def gpu_pauli(n=30, reps=3, seed=9):
     terms over 3 x masks (32 Z-only strings, 32 with X/Y on qubits 0 and
     n-1, 32 with X/Y on every qubit), read-only FP64 passes of up to 16
     terms sharing an x mask; ms per call, per pass and GB/s of the passes
-    (s 2^n bytes each)."""
+    (s 2^n bytes each). Returns ((x masks, z masks), result)."""
     from qcsim_amd import QubitRegister
     from qcsim_amd import gates as G
 
@@ -350,7 +350,7 @@ def gpu_pauli(n=30, reps=3, seed=9):
     ms = (time.perf_counter() - t0) / reps * 1e3
     reg.close()
     passes = 3 * -(-32 // 16)  # per x mask, 16 terms per pass
-    return {
+    return (xs, zs), {
         "qubits": n,
         "terms": len(xs),
         "passes": passes,
@@ -364,28 +364,33 @@ def gpu_pauli(n=30, reps=3, seed=9):
 
 def gpu_vqe_batch(ninst=1024, seed=21, reps=5):
     """Batched VQE: `ninst` registers with n uniform over 4..12 qubits, each a
-    random Hamiltonian of 24 Pauli strings (a quarter of them diagonal) and
+    random Hamiltonian of 24 full-weight Pauli strings (a quarter of them
+    Z-only) and
     4n ansatz parameters (two Ry/Rz layers and a CNOT ladder); exact FP64
-    energies in one launch per size class. Median of `reps` evaluations."""
+    energies in one launch per size class. Median of `reps` evaluations.
+    Returns ((instances, parameters, energies), result)."""
     from qcsim_amd.vqe import VqeBatch
 
     rng = np.random.default_rng(seed)
     insts, params = [], []
     for _ in range(ninst):
         n = int(rng.integers(4, 13))
+        full = (1 << n) - 1
         xs = [0 if k % 4 == 0 else int(v) for k, v in enumerate(rng.integers(0, 1 << n, size=24))]
-        zs = [int(v) for v in rng.integers(0, 1 << n, size=24)]
+        # every qubit X, Y or Z, as the reference's PauliString holds them
+        # (its estimator takes the parity of all qubits)
+        zs = [int(v) | (full & ~x) for v, x in zip(rng.integers(0, 1 << n, size=24), xs)]
         insts.append((n, (xs, zs, [float(v) for v in rng.normal(size=24)])))
         params.append(rng.uniform(0, np.pi, size=4 * n))
     b = VqeBatch(insts)
-    b.evaluate(params)
+    energies = b.evaluate(params)
     ms = []
     for _ in range(reps):
         b.evaluate(params)
         ms.append(b.last_ms())
     b.close()
     t = float(np.median(ms))
-    return {
+    return (insts, params, energies), {
         "instances": ninst,
         "qubits": "4..12",
         "terms_per_instance": 24,
@@ -678,6 +683,97 @@ def cpu_baseline(n, targets, seed=7, reps=5, single_targets=(15,), single_reps=3
     }
 
 
+def _basis_change(term_x, term_z, n):
+    """VQE.h:33-44: Ry(-pi/2) on the string's X qubits, Rx(pi/2) on its Y
+    qubits (x and z bit), as (gate, qubit) pairs."""
+    from qcsim_amd import gates as G
+
+    out = []
+    for q in range(n):
+        if (term_x >> q) & 1:
+            out.append((G.RxGate(0.5 * np.pi) if (term_z >> q) & 1 else G.RyGate(-0.5 * np.pi), q))
+    return out
+
+
+def cpu_vqe_reference(insts, params, exact, per_n=2, shots=10000, seed=23):
+    """The reference's sampled VQE energy (VariationalQuantumEigensolver::
+    EstimateEnergy, VQE.h:204-212, over PauliStringVQE::EstimateEnergy,
+    :116-142: per term |0>, the ansatz, the basis change, `shots` samples of
+    all qubits, the parity average) on the oracle, for `per_n` instances of
+    each size; the batch time is estimated from the per-size means and the
+    batch's size mix. The sampled energies are compared with the engine's
+    exact ones in units of sqrt(sum c^2 / shots), a bound on their standard
+    deviation."""
+    from oracle.oracle import OracleRegister
+    from qcsim_amd.vqe import ansatz_gates
+
+    rng = np.random.default_rng(seed)
+    by_n, zmax = {}, 0.0
+    for i, (n, (xs, zs, cs)) in enumerate(insts):
+        if len(by_n.get(n, [])) >= per_n:
+            continue
+        r = OracleRegister(n)
+        zero = np.zeros(1 << n, dtype=np.complex128)
+        zero[0] = 1.0
+        parity = (np.bitwise_count(np.arange(1 << n, dtype=np.uint64)) & 1).astype(np.float64)
+        seq = ansatz_gates(n, params[i])
+        t0 = time.perf_counter()
+        e = 0.0
+        for x, z, c in zip(xs, zs, cs):
+            r.set_state(zero)
+            for g, qs in seq:
+                r.apply_gate(g, *qs)
+            for g, q in _basis_change(x, z, n):
+                r.apply_gate(g, q)
+            out = r.sample(rng.random(shots))
+            e += c * float(np.mean(1.0 - 2.0 * parity[out.astype(np.int64)]))
+        by_n.setdefault(n, []).append(time.perf_counter() - t0)
+        r.close()
+        zmax = max(zmax, abs(e - float(exact[i])) / (np.sqrt(np.sum(np.square(cs)) / shots)))
+    counts = {}
+    for n, _ in insts:
+        counts[n] = counts.get(n, 0) + 1
+    est = sum(counts[n] * float(np.mean(by_n[n])) for n in counts)
+    return {
+        "kind": "port",
+        "batch_s_estimate": round(est, 3),
+        "evals_per_s": round(len(insts) / est, 1),
+        "shots_per_term": shots,
+        "sample": f"{per_n} instances per size 4..12, the reference's sampled estimator on the oracle "
+        f"(ansatz, basis change and {shots} samples per term), scaled by the batch's size mix",
+        "max_sampled_vs_exact_z": round(zmax, 2),
+    }
+
+
+def cpu_pauli_reference(n, xs, zs, gate_s, shots=10000, seed=29):
+    """Per-term cost of the reference's sampled estimate of a Pauli string on
+    an n-qubit state (PauliStringVQE::EstimateEnergy, VQE.h:116-142, without
+    re-preparing the state): one oracle gate per X/Y qubit of the basis
+    change (`gate_s` each, the CPU baseline's per-gate time) plus one
+    `shots`-sample RepeatedMeasure of all qubits, timed on the oracle."""
+    from oracle.oracle import OracleRegister
+
+    rng = np.random.default_rng(seed)
+    r = OracleRegister(n)
+    u = rng.random(shots)
+    r.sample(u)  # warm-up
+    t0 = time.perf_counter()
+    r.sample(u)
+    t_sample = time.perf_counter() - t0
+    r.close()
+    gates = sum(bin(x).count("1") for x in xs)
+    est = gates * gate_s + len(xs) * t_sample
+    return {
+        "kind": "port",
+        "call_s_estimate": round(est, 2),
+        "sample_s": round(t_sample, 3),
+        "basis_change_gates": gates,
+        "sample": f"{len(xs)} terms: {gates} basis-change gates at {gate_s * 1e3:.0f} ms (the CPU baseline's "
+        f"per-gate time) plus one {shots}-sample RepeatedMeasure per term, timed on the oracle at {n} "
+        "qubits; re-preparing the state per term, as the reference does, is not counted",
+    }
+
+
 def progress(text):
     """One line on stderr per finished leg (stdout carries only the result)."""
     print(f"bench: {text}", file=sys.stderr, flush=True)
@@ -717,8 +813,8 @@ def main():
             "precision": lambda: gpu_precision_fused_c64(args.precision_qubits or 32, args.precision_layers),
             "qaoa": lambda: gpu_qaoa_batch(args.qaoa_instances),
             "learned": gpu_learned_qaoa,
-            "pauli": lambda: gpu_pauli(n),
-            "vqe": gpu_vqe_batch,
+            "pauli": lambda: gpu_pauli(n)[1],
+            "vqe": lambda: gpu_vqe_batch()[1],
         }
         print(json.dumps({"leg": args.only, "result": legs[args.only]()}))
         return
@@ -731,9 +827,9 @@ def main():
     progress(f"sweep: {gates_per_s:.1f} gates/s, {achieved:.0f} GB/s")
     qft = gpu_qft(n)
     progress(f"qft: {qft['ms']} ms")
-    pauli = gpu_pauli(n)
+    pauli_terms, pauli = gpu_pauli(n)
     progress(f"pauli leg: {pauli['ms_per_pass']} ms per pass")
-    vqe = gpu_vqe_batch()
+    vqe_inputs, vqe = gpu_vqe_batch()
     progress(f"batched VQE leg: {vqe['batch_ms']} ms per batch")
     sampling, c0_seq, c0_u, c0_psi, c0_out = gpu_sampling()
     progress(f"sampling leg done: min CDF margin {sampling['min_cdf_margin']:.2e}")
@@ -763,6 +859,10 @@ def main():
                 "sample": "one instance per size, gate-by-gate FP64 circuit plus energy: "
                 + ", ".join(f"n={k}: {v * 1e3:.1f} ms" for k, v in per_n.items()),
             }
+        pauli["cpu_reference"] = cpu_pauli_reference(n, *pauli_terms, gate_s=1.0 / cpu["value"])
+        progress("cpu reference of the Pauli leg done")
+        vqe["cpu_reference"] = cpu_vqe_reference(*vqe_inputs)
+        progress("cpu reference of the batched VQE leg done")
     out = {
         "metric": "gate/s & HBM GB/s @30q fp64",
         "value": round(gates_per_s, 3),
