@@ -97,7 +97,10 @@ int qcs_create(uint32_t nqubits, int precision, qcs_register** out) {
     r->n = nqubits;
     r->A = 1ull << nqubits;
     r->prec = precision;
-    hipGetDevice(&r->device);
+    if (hipGetDevice(&r->device) != hipSuccess) {
+        delete r;
+        return set_error(QCS_EDEVICE, "no HIP device");
+    }
     const size_t bytes = r->A * r->amp_bytes();
     const uint64_t nchunks = r->A > (1ull << kChunkLog) ? (r->A >> kChunkLog) : 1;
     const uint64_t cap = device_cap();
@@ -106,22 +109,22 @@ int qcs_create(uint32_t nqubits, int precision, qcs_register** out) {
         return set_error(QCS_ENOMEM, "state of %zu bytes exceeds QCS_MAX_DEVICE_BYTES", bytes);
     }
     auto fail = [&](int code, const char* what) {
-        if (r->psi) hipFree(r->psi);
-        if (r->red) hipFree(r->red);
-        if (r->chunk) hipFree(r->chunk);
-        if (r->dout) hipFree(r->dout);
-        if (r->mat) hipFree(r->mat);
-        if (r->xfer) hipFree(r->xfer);
-        if (r->hmat) hipHostFree(r->hmat);
-        if (r->hpin) hipHostFree(r->hpin);
+        if (r->psi) (void)hipFree(r->psi);
+        if (r->red) (void)hipFree(r->red);
+        if (r->chunk) (void)hipFree(r->chunk);
+        if (r->dout) (void)hipFree(r->dout);
+        if (r->mat) (void)hipFree(r->mat);
+        if (r->xfer) (void)hipFree(r->xfer);
+        if (r->hmat) (void)hipHostFree(r->hmat);
+        if (r->hpin) (void)hipHostFree(r->hpin);
         for (hipEvent_t ev : r->mat_ev)
-            if (ev) hipEventDestroy(ev);
-        if (r->stream) hipStreamDestroy(r->stream);
+            if (ev) (void)hipEventDestroy(ev);
+        if (r->stream) (void)hipStreamDestroy(r->stream);
         delete r;
         return set_error(code, "%s", what);
     };
     if (hipMalloc(&r->psi, bytes) != hipSuccess) {
-        hipGetLastError();
+        (void)hipGetLastError();
         return fail(QCS_ENOMEM, "hipMalloc of the state failed");
     }
     r->mat_bytes = sizeof(double) * 64 * 64;  // real embedding of a 32x32 complex block
@@ -130,7 +133,7 @@ int qcs_create(uint32_t nqubits, int precision, qcs_register** out) {
             hipSuccess ||
         hipMalloc(reinterpret_cast<void**>(&r->dout), sizeof(uint64_t) * 64) != hipSuccess ||
         hipMalloc(&r->mat, r->mat_bytes * qcs_register::kMatSlots) != hipSuccess) {
-        hipGetLastError();
+        (void)hipGetLastError();
         return fail(QCS_ENOMEM, "hipMalloc of scratch failed");
     }
     if (hipHostMalloc(reinterpret_cast<void**>(&r->hpin), sizeof(double) * 64) != hipSuccess ||
@@ -156,20 +159,20 @@ int qcs_create(uint32_t nqubits, int precision, qcs_register** out) {
 
 int qcs_destroy(qcs_register* r) {
     if (!r) return QCS_OK;
-    if (r->stream) hipStreamSynchronize(r->stream);
-    hipFree(r->psi);
-    hipFree(r->red);
-    hipFree(r->chunk);
-    hipFree(r->dout);
-    hipFree(r->mat);
-    if (r->xfer) hipFree(r->xfer);
-    hipHostFree(r->hmat);
-    hipHostFree(r->hpin);
+    if (r->stream) (void)hipStreamSynchronize(r->stream);
+    (void)hipFree(r->psi);
+    (void)hipFree(r->red);
+    (void)hipFree(r->chunk);
+    (void)hipFree(r->dout);
+    (void)hipFree(r->mat);
+    if (r->xfer) (void)hipFree(r->xfer);
+    (void)hipHostFree(r->hmat);
+    (void)hipHostFree(r->hpin);
     for (hipEvent_t ev : r->mat_ev)
-        if (ev) hipEventDestroy(ev);
-    if (r->ev0) hipEventDestroy(r->ev0);
-    if (r->ev1) hipEventDestroy(r->ev1);
-    if (r->stream) hipStreamDestroy(r->stream);
+        if (ev) (void)hipEventDestroy(ev);
+    if (r->ev0) (void)hipEventDestroy(r->ev0);
+    if (r->ev1) (void)hipEventDestroy(r->ev1);
+    if (r->stream) (void)hipStreamDestroy(r->stream);
     delete r;
     return QCS_OK;
 }

@@ -570,7 +570,10 @@ static int rising_t(qcs_register* r, uint32_t ne, const uint32_t* ei, const uint
         hi[k] = ei[k];
         hi[ne + k] = ej[k];
     }
-    hipMemcpyAsync(d, hbuf.data(), bytes, hipMemcpyHostToDevice, r->stream);
+    if (hipMemcpyAsync(d, hbuf.data(), bytes, hipMemcpyHostToDevice, r->stream) != hipSuccess) {
+        (void)hipFreeAsync(d, r->stream);
+        return set_error(QCS_EDEVICE, "ising args upload");
+    }
     const double* dh = reinterpret_cast<const double*>(d);
     const uint32_t* dij = reinterpret_cast<const uint32_t*>(d + sizeof(double) * (r->n + ne));
     const int nb = red_blocks(r->A);
@@ -578,11 +581,12 @@ static int rising_t(qcs_register* r, uint32_t ne, const uint32_t* ei, const uint
     hipLaunchKernelGGL((k_ising<T>), dim3(nb), dim3(kRedThreads), 0, r->stream, static_cast<const cv<T>*>(r->psi),
                        r->A, ia, dh, dij, dij + ne, dh + r->n, r->red);
     int e = check_launch(r, "k_ising");
-    if (e) return e;
     double* res = r->red + kRedResult;
-    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(kRedThreads), 0, r->stream, r->red, nb, 1, res);
-    e = check_launch(r, "k_final_sum");
-    hipFreeAsync(d, r->stream);
+    if (!e) {
+        hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(kRedThreads), 0, r->stream, r->red, nb, 1, res);
+        e = check_launch(r, "k_final_sum");
+    }
+    (void)hipFreeAsync(d, r->stream);
     if (e) return e;
     return fetch(r, res, 1, energy);
 }
