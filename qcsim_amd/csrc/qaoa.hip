@@ -274,7 +274,7 @@ __device__ __forceinline__ void rxs_reg(float2* x, const RxForm& r) {
     else rxs_reg_f<G, false>(x, r.k);
 }
 
-enum : int { IN_PLUS = 0, IN_GLOBAL = 1, IN_LDS = 2 };
+enum : int { IN_PLUS = 0, IN_REG = 1, IN_LDS = 2 };
 
 // Rx(2 beta) on the G tile qubits lo..lo+G-1, each thread (index tid) taking
 // orbits of 2^G elements from the LDS tile into registers.
@@ -439,7 +439,7 @@ __device__ void thread_terms(ThreadTerms& W, double gamma, const QInst& I, const
 template <int M>
 __device__ void tile_layer(float2* t, uint32_t tid, uint32_t jt, double gamma, const RxForm& rf, const QInst& I,
                            const IsingLds& P, TileCostLds& TC, const ThreadTerms& W, int in_mode,
-                           const float2* __restrict__ in, uint32_t ci, float amp) {
+                           const float2 (&xin)[1 << M], float amp) {
     constexpr uint32_t tb = 8 + M;
     // per-thread terms, from the tile's terms (tile_terms, slot jt)
     double A = TC.K[jt] + W.c0;
@@ -474,20 +474,24 @@ __device__ void tile_layer(float2* t, uint32_t tid, uint32_t jt, double gamma, c
         const float2 qm = TC.q[gm];
         const float pr = ph.x, pi = ph.y;
         const float wr = pr * qm.x - pi * qm.y, wi = pr * qm.y + pi * qm.x;
-        const uint32_t i = tid + (gm << 8);
-        const float2 v = in_mode == IN_GLOBAL ? ldamp<float, true>(in + (i ^ ci))
-                                              : (in_mode == IN_LDS ? t[swz(tid) + swz(gm << 8)] : make_float2(amp, 0.f));
+        const float2 v = in_mode == IN_REG ? xin[gm]
+                                           : (in_mode == IN_LDS ? t[swz(tid) + swz(gm << 8)] : make_float2(amp, 0.f));
         y[gm] = make_float2(wr * v.x - wi * v.y, wr * v.y + wi * v.x);
     }
     // y is the orbit of qubits 8..tb-1: their Rx before the LDS store
     if constexpr (M > 0) rxs_reg<M>(y, rf);
     // every wave has read what it needs of the tile in LDS (the previous
-    // tile's store, or this layer's input) before it is overwritten; the
-    // global loads above are already in flight or done
+    // tile's store, or this layer's input) before it is overwritten
     __syncthreads();
 #pragma unroll
     for (uint32_t m = 0; m < (1u << M); ++m) t[swz(tid) + swz(m << 8)] = y[m];
     __syncthreads();
+}
+
+// The rest of a layer on the tile in LDS: Rx on qubits 4..7 and 0..3.
+template <int M>
+__device__ void tile_rounds(float2* t, uint32_t tid, const RxForm& rf) {
+    constexpr uint32_t tb = 8 + M;
     if constexpr (M == 4) {
         // wave-local rounds; the caller orders the result before other waves
         // read it
@@ -502,7 +506,13 @@ __device__ void tile_layer(float2* t, uint32_t tid, uint32_t jt, double gamma, c
 }
 
 // A tile unit: one layer on tiles tile0 .. tile0+ntile-1 of an instance,
-// 2^(8+M) amplitudes each, written back to psi.
+// 2^(8+M) amplitudes each, written back to psi. Software-pipelined: the next
+// tile's global loads are issued as soon as this tile is in LDS (its input
+// registers are then free), so that they are in flight during this tile's
+// LDS rounds and stores and the next tile's per-thread terms. The first tile
+// is peeled off the loop, so that the loop head is reached with the same
+// memory operations outstanding on every path (the next tile's loads, then
+// this tile's stores) and waits only for the loads.
 template <int M>
 __device__ void tile_run(float2* t, float2* psi, uint64_t tile0, uint32_t ntile, double gamma, double c, double s,
                          const QInst& I, const IsingLds& P, TileCostLds& TC, bool first, float amp) {
@@ -511,19 +521,35 @@ __device__ void tile_run(float2* t, float2* psi, uint64_t tile0, uint32_t ntile,
     ThreadTerms W;
     tile_terms<M>(TC, gamma, I, P, tile0, ntile);
     thread_terms<M>(W, gamma, I, P, TC);
-    for (uint32_t j = 0; j < ntile; ++j) {
+    // stored positions: element i of the tile at base is io[i ^ ci] (io =
+    // psi + base for tiles of >= 1024 amplitudes; a smaller tile trades
+    // elements with the tile at base ^ (the mask's bits >= tb), which has
+    // the same mask)
+    auto io_of = [&](uint64_t base) {
+        const uint64_t cx = arena_xor(base);
+        return psi + (base ^ (cx & ~uint64_t(C - 1)));
+    };
+    auto ci_of = [&](uint64_t base) { return uint32_t(arena_xor(base)) & (C - 1); };
+    float2 xin[1 << M];
+    auto fetch = [&](uint64_t base) {
+        uint32_t tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        __builtin_assume(tid < 256u);
+        const float2* io = io_of(base);
+        const uint32_t ci = ci_of(base);
+#pragma unroll
+        for (uint32_t m = 0; m < (1u << M); ++m) xin[m] = ldamp<float, true>(io + ((tid + (m << 8)) ^ ci));
+    };
+    auto one_tile = [&](uint32_t j) {
         const uint64_t base = (tile0 + j) << tb;
         uint32_t tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
         __builtin_assume(tid < 256u);
-        // stored positions: element i of the tile at base is io[i ^ ci]
-        // (io = psi + base for tiles of >= 1024 amplitudes; a smaller tile
-        // trades elements with the tile at base ^ (the mask's bits >= tb),
-        // which has the same mask)
-        const uint64_t cx = arena_xor(base);
-        float2* const io = psi + (base ^ (cx & ~uint64_t(C - 1)));
-        const uint32_t ci = uint32_t(cx) & (C - 1);
-        tile_layer<M>(t, tid, j, gamma, rf, I, P, TC, W, first ? IN_PLUS : IN_GLOBAL, io, ci, amp);
+        tile_layer<M>(t, tid, j, gamma, rf, I, P, TC, W, first ? IN_PLUS : IN_REG, xin, amp);
+        if (!first && j + 1 < ntile) fetch(base + C);
+        tile_rounds<M>(t, tid, rf);
+        float2* const io = io_of(base);
+        const uint32_t ci = ci_of(base);
         if constexpr (M == 4) {
             // each wave stores the 1024 elements it rotated last
 #pragma unroll
@@ -537,7 +563,10 @@ __device__ void tile_run(float2* t, float2* psi, uint64_t tile0, uint32_t ntile,
                 stamp<float, true>(io + ((tid + 256 * r) ^ ci), t[swz(tid) + swz(256 * r)]);
             }
         }
-    }
+    };
+    if (!first) fetch(tile0 << tb);
+    one_tile(0);
+    for (uint32_t j = 1; j < ntile; ++j) one_tile(j);
 }
 
 // Every layer of a SMALL instance of 8 + M qubits, the state kept in the LDS
@@ -554,7 +583,10 @@ __device__ void small_run(float2* t, const double* an, uint32_t p, const QInst& 
         uint32_t tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
         __builtin_assume(tid < 256u);
-        tile_layer<M>(t, tid, 0, an[2 * l], rx_form(cb, sb, 8 + M), I, P, TC, W, l ? IN_LDS : IN_PLUS, nullptr, 0, amp);
+        const RxForm rf = rx_form(cb, sb, 8 + M);
+        float2 none[1 << M] = {};  // not read: the input is |+> or the LDS tile
+        tile_layer<M>(t, tid, 0, an[2 * l], rf, I, P, TC, W, l ? IN_LDS : IN_PLUS, none, amp);
+        tile_rounds<M>(t, tid, rf);
     }
     __syncthreads();  // the energy pass reads every wave's elements
 }
@@ -583,13 +615,14 @@ __device__ __forceinline__ void small_run_tb(uint32_t tb, float2* t, const doubl
 }
 
 // Tile units (SMALL = false): one layer on a run of 2^T tiles. SMALL units:
-// every layer and the energy of an instance that fits one tile. Four waves
-// per SIMD (at most 128 VGPRs): four blocks per CU is what the LDS tile
-// allows at T = 12, and the kernel is bound by its VALU issue rate, which
-// needs the waves more than the registers (three waves per SIMD, with no
-// spills, measured 4 % slower per batch).
+// every layer and the energy of an instance that fits one tile. Three waves
+// per SIMD (at most 168 VGPRs): the software-pipelined tile loop keeps the
+// next tile's 2^M loads in registers across the LDS rounds, and at four
+// waves (128 VGPRs) it spills inside the loop, where a scratch reload waits
+// for every outstanding load and store (measured 1 % slower per batch than
+// three waves; unpipelined, four and three waves measured alike).
 template <bool SMALL>
-__global__ __launch_bounds__(256, 4) void k_qaoa_tile(float2* __restrict__ arena, const QInst* __restrict__ insts,
+__global__ __launch_bounds__(256, 3) void k_qaoa_tile(float2* __restrict__ arena, const QInst* __restrict__ insts,
                                                    const QUnit* __restrict__ units, uint32_t nunits,
                                                    uint32_t* __restrict__ ticket, const double* __restrict__ ang,
                                                    uint32_t p, IsingArgsDev ga, double* __restrict__ energy) {
