@@ -9,7 +9,11 @@ gates/s; the roofline leg prices each launch at the algorithmic
 
 cpu_baseline: the CPU oracle (C++17 + OpenMP restatement of the reference's
 loops, results buffer + swap) on the host cores, on a bounded sample of the
-same workload (30 qubits, a few target qubits), reported per gate.
+same workload (30 qubits, a few target qubits), reported per gate. The same
+CPU leg (skipped with --no-cpu) also times the oracle on bounded samples of
+the other legs' workloads (the sampling workload, the batched QAOA, the
+Pauli strings and the batched VQE); it is the only part of this file that
+calls the oracle.
 """
 import argparse
 import json
