@@ -97,6 +97,28 @@ def test_config_sizes_match_oracle(tile_bits, group):
     batch.close()
 
 
+def test_26_qubit_instance_matches_oracle():
+    """An instance above 24 qubits: three mixer passes per layer at tile 12 /
+    group 6 (qubits 12..17, 18..23, 24..25) and a stored order that folds
+    both of its high 6-bit fields (bits 18.. and 24..); p = 2, with a 16-qubit
+    instance in the same batch."""
+    insts = make_batch_instances(2026, [16, 26])
+    p = 2
+    rng = np.random.default_rng(26)
+    G = rng.uniform(0, np.pi, (len(insts), p))
+    B = rng.uniform(0, np.pi, (len(insts), p))
+    batch = QaoaBatch(insts, p, tile_bits=12, group=6)
+    E = batch.evaluate(G, B)
+    for i, inst in enumerate(insts):
+        e_ref, psi_ref = oracle_eval(inst, G[i], B[i])
+        scale = 1 + np.sum(np.abs(inst.J)) + np.sum(np.abs(inst.h))
+        err = float(np.max(np.abs(batch.state(i) - psi_ref)))
+        print(f"n={inst.n}: energy {E[i]:.6f} vs oracle {e_ref:.6f}, state max-abs {err:.2e}")
+        assert abs(E[i] - e_ref) < 2e-5 * scale, (inst.n, E[i], e_ref)
+        assert err < 3e-6, (inst.n, err)
+    batch.close()
+
+
 def test_large_uneven_batch_sample_matches_oracle():
     """A batch of 256 instances with n uniform over 12, 14, ..., 24 at p = 4 (the
     batched-QAOA workload's mix at a quarter of its count), evaluated in one
