@@ -6,8 +6,11 @@
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <atomic>
 #include <functional>
+#include <future>
 #include <iostream>
+#include <random>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -569,6 +572,69 @@ static bool recordedQftUncompute() {
     return true;
 }
 
+// CliffordTests.cpp:208-258: registers created, driven and measured on many
+// std::async threads at once (one register per shot), each applying the same
+// random Clifford+T circuit. Every thread's state before the measurements
+// must equal the one computed alone, and measuring every qubit must leave a
+// basis state.
+static bool concurrentRegisters() {
+    const size_t n = 10, ngates = 200;
+    std::mt19937_64 gen(5);
+    std::uniform_int_distribution<int> kind(0, 7);
+    std::uniform_int_distribution<size_t> qd(0, n - 1);
+    std::vector<int> gk(ngates);
+    std::vector<size_t> q1(ngates), q2(ngates);
+    for (size_t j = 0; j < ngates; ++j) {
+        gk[j] = kind(gen);
+        q1[j] = qd(gen);
+        q2[j] = (q1[j] + 1 + qd(gen) % (n - 1)) % n;
+    }
+    auto run = [&](QC::QubitRegister<>& reg) {
+        QC::Gates::HadamardGate<> h;
+        QC::Gates::PauliXGate<> x;
+        QC::Gates::PauliYGate<> y;
+        QC::Gates::PauliZGate<> z;
+        QC::Gates::SGate<> sg;
+        QC::Gates::TGate<> t;
+        QC::Gates::CNOTGate<> cx;
+        QC::Gates::ControlledZGate<> cz;
+        for (size_t j = 0; j < ngates; ++j) {
+            switch (gk[j]) {
+                case 0: reg.ApplyGate(h, q1[j]); break;
+                case 1: reg.ApplyGate(x, q1[j]); break;
+                case 2: reg.ApplyGate(y, q1[j]); break;
+                case 3: reg.ApplyGate(z, q1[j]); break;
+                case 4: reg.ApplyGate(sg, q1[j]); break;
+                case 5: reg.ApplyGate(t, q1[j]); break;
+                case 6: reg.ApplyGate(cx, q1[j], q2[j]); break;
+                default: reg.ApplyGate(cz, q1[j], q2[j]); break;
+            }
+        }
+    };
+    QC::QubitRegister<> alone(n);
+    run(alone);
+    const Eigen::VectorXcd want = alone.getRegisterStorage();
+    std::atomic<int> bad{0};
+    std::vector<std::future<void>> tasks;
+    for (int th = 0; th < 8; ++th)
+        tasks.push_back(std::async(std::launch::async, [&]() {
+            for (int shot = 0; shot < 4; ++shot) {
+                QC::QubitRegister<> reg(n);
+                run(reg);
+                const Eigen::VectorXcd got = reg.getRegisterStorage();
+                for (size_t i = 0; i < (size_t(1) << n); ++i)
+                    if (!approx(got(i), want(i), 1e-13)) ++bad;
+                size_t v = 0;
+                for (size_t q = 0; q < n; ++q)
+                    if (reg.MeasureQubit(q)) v |= size_t(1) << q;
+                const Eigen::VectorXcd post = reg.getRegisterStorage();
+                if (std::abs(std::abs(post(v)) - 1.0) > 1e-12) ++bad;
+            }
+        }));
+    for (auto& t : tasks) t.get();
+    return bad.load() == 0;
+}
+
 int main() {
     const std::vector<std::pair<const char*, bool (*)()>> tests = {
         {"measurementCollapse", measurementCollapse},
@@ -587,6 +653,7 @@ int main() {
         {"algorithmInterface", algorithmInterface},
         {"operatorsAndPauli", operatorsAndPauli},
         {"recordedQftUncompute", recordedQftUncompute},
+        {"concurrentRegisters", concurrentRegisters},
     };
     int failed = 0;
     for (const auto& t : tests) {
