@@ -329,6 +329,10 @@ public:
         return mirror_;
     }
 
+    // The same, with the capitalised spelling some callers use (the reference
+    // spells it getRegisterStorage, QubitRegister.h:507).
+    const VectorClass& GetRegisterStorage() const { return getRegisterStorage(); }
+
     void setRegisterStorage(const VectorClass& vals) {
         if (size_t(vals.size()) != NrBasisStates) return;
         upload(vals);

@@ -129,6 +129,8 @@ class QubitRegister:
         check(lib().qcs_get_state(self._h, _dptr(out.view(np.float64)), self.NrBasisStates))
         return out
 
+    GetRegisterStorage = getRegisterStorage  # capitalised spelling (the reference's is lower-case)
+
     def setRegisterStorage(self, vals):
         v = np.ascontiguousarray(vals, dtype=np.complex128)
         if v.shape[0] != self.NrBasisStates:

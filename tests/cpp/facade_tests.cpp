@@ -250,6 +250,7 @@ static bool stateManagement() {
     reg.ApplyGate(h, 0);
     reg.RestoreState();
     if (!approx(reg.getBasisStateAmplitude(4), 1. / std::sqrt(2.))) return false;
+    if (&reg.GetRegisterStorage() != &reg.getRegisterStorage()) return false;  // the alias
     auto c = reg.Clone();
     if (!approx(c->getBasisStateAmplitude(4), 1. / std::sqrt(2.))) return false;
     Eigen::VectorXcd v = Eigen::VectorXcd::Zero(16);
