@@ -7,13 +7,16 @@
 // it; the block then holds every such gate, in their original order. Since
 // each remaining gate that precedes a block gate acts on other qubits, the
 // two commute and the block order is a valid reordering of the sequence.
-// S starts as the qubits of the first pending gate and grows greedily: each
-// step tries the qubit sets of the pending gates that are not blocked by an
-// earlier gate (the frontier), keeps the union within W qubits that takes the
-// most gates (fewest new qubits on ties), and stops when no union adds one.
-// On layered random circuits this gives 40 % fewer blocks than growing
-// blocks in gate order (30 qubits, 60 layers of U(2) + random pairs: 217 for
-// W = 5 against 381).
+// S starts as the qubits of a pending gate that no earlier pending gate
+// blocks and grows greedily: each step tries the qubit sets of the frontier
+// (the pending gates not blocked by an earlier one), keeps the union within W
+// qubits that takes the most gates (fewest new qubits on ties), and stops
+// when no union adds one. Each block tries the first four such seeds, and
+// for each also every one-step extension grown the same way, and keeps the
+// candidate that takes the most gates. On layered random circuits this gives
+// 45 % fewer blocks than growing blocks in gate order (30 qubits, 60 layers
+// of U(2) + CNOT / CZ on random pairs: 211 for W = 5 against about 380; 215
+// when blocks start at the first pending gate only, without the extensions).
 //
 // Each gate enters the product with its *effective* matrix: the matrix that
 // the reference's trait-driven kernels apply (QubitRegisterCalculator.h:
@@ -168,6 +171,8 @@ struct Pending {
 // left out. Fills `take` (gate indices) and `front` (qubit masks of gates
 // left out that no earlier gate blocks) when given.
 constexpr int kScanLimit = 4096;
+// block seeds tried per block (first unblocked pending gates)
+constexpr int kFuseSeeds = 4;
 
 static int scan(const Pending& P, uint64_t S, std::vector<int>* take, std::vector<uint64_t>* front) {
     uint64_t blocked = 0;
@@ -207,29 +212,68 @@ int fuse(const qcs_gate* gates, uint32_t n, uint32_t width, std::vector<qcs_gate
     std::vector<std::vector<uint32_t>> blocks;  // gate indices per block, in order
     std::vector<uint64_t> front;
     std::vector<int> take;
+    // grows S greedily: each step adds the frontier qubit set that takes the
+    // most gates (fewest new qubits on ties), until none adds a gate
+    auto grow = [&](uint64_t S) {
+        for (;;) {
+            front.clear();
+            const int have = scan(P, S, nullptr, &front);
+            std::sort(front.begin(), front.end());
+            front.erase(std::unique(front.begin(), front.end()), front.end());
+            int best = have, best_add = 99;
+            uint64_t bestS = S;
+            for (const uint64_t f : front) {
+                const uint64_t S2 = S | f;
+                const int add = popc(f & ~S);
+                if (popc(S2) > int(width) || S2 == S) continue;
+                const int c = scan(P, S2, nullptr, nullptr);
+                if (c > best || (c == best && c > have && add < best_add)) {
+                    best = c;
+                    best_add = add;
+                    bestS = S2;
+                }
+            }
+            if (best == have) return std::make_pair(S, have);
+            S = bestS;
+        }
+    };
+    std::vector<uint64_t> seeds, fr;
     while (P.head >= 0) {
         uint64_t S = P.mask[P.head];
         if (gates[P.head].k <= width) {
-            for (;;) {
+            // seeds: the qubit sets of the first kFuseSeeds pending gates that
+            // no earlier pending gate blocks (each could open a block); each
+            // seed is grown greedily, and so is each of its one-step
+            // extensions, and the block that takes the most gates is kept
+            seeds.clear();
+            uint64_t blocked = 0;
+            for (int i = P.head; i >= 0 && int(seeds.size()) < kFuseSeeds; i = P.next[i]) {
+                const uint64_t m = P.mask[i];
+                if (!(m & blocked) && popc(m) <= int(width)) seeds.push_back(m);
+                blocked |= m;
+                if (blocked == ~0ull) break;
+            }
+            int best = -1;
+            for (const uint64_t s0 : seeds) {
+                const auto r = grow(s0);
+                if (r.second > best) {
+                    best = r.second;
+                    S = r.first;
+                }
                 front.clear();
-                const int have = scan(P, S, nullptr, &front);
-                std::sort(front.begin(), front.end());
-                front.erase(std::unique(front.begin(), front.end()), front.end());
-                int best = have, best_add = 99;
-                uint64_t bestS = S;
-                for (const uint64_t f : front) {
-                    const uint64_t S2 = S | f;
-                    const int add = popc(f & ~S);
-                    if (popc(S2) > int(width) || S2 == S) continue;
-                    const int c = scan(P, S2, nullptr, nullptr);
-                    if (c > best || (c == best && c > have && add < best_add)) {
-                        best = c;
-                        best_add = add;
-                        bestS = S2;
+                scan(P, s0, nullptr, &front);
+                fr.assign(front.begin(), front.end());
+                std::sort(fr.begin(), fr.end());
+                fr.erase(std::unique(fr.begin(), fr.end()), fr.end());
+                for (const uint64_t f : fr) {
+                    const uint64_t S2 = s0 | f;
+                    if (popc(S2) > int(width) || S2 == s0) continue;
+                    const auto r2 = grow(S2);
+                    if (r2.second > best) {
+                        best = r2.second;
+                        S = r2.first;
                     }
                 }
-                if (best == have) break;
-                S = bestS;
             }
             take.clear();
             scan(P, S, &take, nullptr);
