@@ -19,6 +19,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <functional>
 #include <vector>
 
 #include "common.h"
@@ -46,6 +47,7 @@ int check_launch(qcs_register* r, const char* what) {
 
 int qft_sequence(qcs_register* r, uint32_t sq, uint32_t eq, int do_swap, int inverse);
 int fuse(const qcs_gate* gates, uint32_t n, uint32_t width, std::vector<qcs_gate>& out);
+int fuse_stream(const qcs_gate* gates, uint32_t n, uint32_t width, const std::function<int(const qcs_gate&)>& emit);
 
 int stage_upload(qcs_register* r, const void* src, size_t bytes, const void** dev) {
     if (bytes > r->mat_bytes) return set_error(QCS_EINVAL, "parameter block too large");
@@ -366,28 +368,31 @@ int qcs_apply_circuit(qcs_register* r, const qcs_gate* gates, uint32_t n, uint32
     QCS_REQUIRE(r && (gates || n == 0), "null argument");
     QCS_REQUIRE(fuse_width <= 5 || fuse_width == QCS_FUSE_AUTO, "fusion width must be 0..5 or QCS_FUSE_AUTO");
     r->last_fuse_width = fuse_width == QCS_FUSE_AUTO ? 0 : fuse_width;
-    std::vector<qcs_gate> fused;
-    const qcs_gate* seq = gates;
-    uint32_t count = n;
-    if (fuse_width >= 2) {
-        for (uint32_t i = 0; i < n; ++i)
-            for (uint32_t j = 0; j < gates[i].k && j < 5; ++j)
-                QCS_REQUIRE(gates[i].qubits[j] < r->n, "gate %u: qubit out of range", i);
-        uint32_t picked = fuse_width;
-        const int e = fuse_width == QCS_FUSE_AUTO ? pick_fusion(r, gates, n, fused, &picked)
-                                                  : fuse(gates, n, fuse_width, fused);
-        if (e) return e;
-        r->last_fuse_width = picked;
-        seq = fused.data();
-        count = uint32_t(fused.size());
-    }
-    for (uint32_t i = 0; i < count; ++i) {
-        const qcs_gate& g = seq[i];
+    auto apply = [r](const qcs_gate& g) {
         const bool dense = (g.flags & QCS_GATE_DENSE) || g.k > 3;
-        const int e = dense ? qcs_apply_dense(r, g.k, g.qubits, g.mat)
-                            : qcs_apply_gate(r, g.k, g.traits, g.qubits[0], g.qubits[1], g.qubits[2], g.mat);
-        if (e) return e;
+        return dense ? qcs_apply_dense(r, g.k, g.qubits, g.mat)
+                     : qcs_apply_gate(r, g.k, g.traits, g.qubits[0], g.qubits[1], g.qubits[2], g.mat);
+    };
+    if (fuse_width < 2) {
+        for (uint32_t i = 0; i < n; ++i)
+            if (const int e = apply(gates[i])) return e;
+        return QCS_OK;
     }
+    for (uint32_t i = 0; i < n; ++i)
+        for (uint32_t j = 0; j < gates[i].k && j < 5; ++j)
+            QCS_REQUIRE(gates[i].qubits[j] < r->n, "gate %u: qubit out of range", i);
+    if (fuse_width != QCS_FUSE_AUTO) {
+        // each block is launched as soon as the fusion pass has built it, so
+        // that the rest of the pass runs on the host while the GPU applies it
+        r->last_fuse_width = fuse_width;
+        return fuse_stream(gates, n, fuse_width, apply);
+    }
+    std::vector<qcs_gate> fused;
+    uint32_t picked = 0;
+    if (const int e = pick_fusion(r, gates, n, fused, &picked)) return e;
+    r->last_fuse_width = picked;
+    for (const qcs_gate& g : fused)
+        if (const int e = apply(g)) return e;
     return QCS_OK;
 }
 

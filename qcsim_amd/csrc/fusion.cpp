@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <complex>
 #include <cstring>
+#include <functional>
 #include <vector>
 
 #include "qcsim_amd.h"
@@ -191,7 +192,33 @@ static int scan(const Pending& P, uint64_t S, std::vector<int>* take, std::vecto
     return count;
 }
 
-int fuse(const qcs_gate* gates, uint32_t n, uint32_t width, std::vector<qcs_gate>& out) {
+// The block of gates `bg` (indices into gates, in sequence order, at least
+// one) as one gate: a single gate stays itself (and keeps its cheaper
+// kernel), several become a dense block on the qubits they act on.
+static qcs_gate block_gate(const qcs_gate* gates, const Pending& P, const std::vector<int>& bg) {
+    if (bg.size() == 1) return gates[bg[0]];
+    uint64_t used = 0;  // S may hold qubits no taken gate acts on
+    for (int gi : bg) used |= P.mask[gi];
+    std::vector<uint32_t> bq;
+    for (uint32_t q = 0; q < 64; ++q)
+        if (used >> q & 1) bq.push_back(q);
+    const int w = int(bq.size()), W = 1 << w;
+    std::vector<cplx> M(size_t(W) * W, cplx(0, 0));
+    for (int d = 0; d < W; ++d) M[size_t(d) * W + d] = 1.0;
+    for (int gi : bg) left_multiply(M, w, bq, gates[gi]);
+    qcs_gate fg;
+    std::memset(&fg, 0, sizeof fg);
+    fg.k = uint32_t(w);
+    fg.flags = QCS_GATE_DENSE;
+    for (int j = 0; j < w; ++j) fg.qubits[j] = bq[j];
+    for (int e = 0; e < W * W; ++e) {
+        fg.mat[2 * e] = M[e].real();
+        fg.mat[2 * e + 1] = M[e].imag();
+    }
+    return fg;
+}
+
+int fuse_stream(const qcs_gate* gates, uint32_t n, uint32_t width, const std::function<int(const qcs_gate&)>& emit) {
     if (width < 1 || width > 5) return set_error(QCS_EINVAL, "fusion width must be 1..5");
     Pending P;
     P.mask.resize(n);
@@ -209,7 +236,6 @@ int fuse(const qcs_gate* gates, uint32_t n, uint32_t width, std::vector<qcs_gate
         P.next[i] = i + 1 < n ? int(i + 1) : -1;
     }
     P.head = n ? 0 : -1;
-    std::vector<std::vector<uint32_t>> blocks;  // gate indices per block, in order
     std::vector<uint64_t> front;
     std::vector<int> take;
     // grows S greedily: each step adds the frontier qubit set that takes the
@@ -294,37 +320,18 @@ int fuse(const qcs_gate* gates, uint32_t n, uint32_t width, std::vector<qcs_gate
                 prev = i;
             }
         }
-        blocks.emplace_back(take.begin(), take.end());
-    }
-    out.clear();
-    out.reserve(blocks.size());
-    for (size_t bi = 0; bi < blocks.size(); ++bi) {
-        const std::vector<uint32_t>& bg = blocks[bi];
-        if (bg.size() == 1) {
-            out.push_back(gates[bg[0]]);
-            continue;
-        }
-        uint64_t used = 0;  // S may hold qubits no taken gate acts on
-        for (uint32_t gi : bg) used |= P.mask[gi];
-        std::vector<uint32_t> bq;
-        for (uint32_t q = 0; q < 64; ++q)
-            if (used >> q & 1) bq.push_back(q);
-        const int w = int(bq.size()), W = 1 << w;
-        std::vector<cplx> M(size_t(W) * W, cplx(0, 0));
-        for (int d = 0; d < W; ++d) M[size_t(d) * W + d] = 1.0;
-        for (uint32_t gi : bg) left_multiply(M, w, bq, gates[gi]);
-        qcs_gate fg;
-        std::memset(&fg, 0, sizeof fg);
-        fg.k = uint32_t(w);
-        fg.flags = QCS_GATE_DENSE;
-        for (int j = 0; j < w; ++j) fg.qubits[j] = bq[j];
-        for (int e = 0; e < W * W; ++e) {
-            fg.mat[2 * e] = M[e].real();
-            fg.mat[2 * e + 1] = M[e].imag();
-        }
-        out.push_back(fg);
+        // the block leaves (ascending indices) as one gate, at once
+        if (const int e = emit(block_gate(gates, P, take))) return e;
     }
     return QCS_OK;
+}
+
+int fuse(const qcs_gate* gates, uint32_t n, uint32_t width, std::vector<qcs_gate>& out) {
+    out.clear();
+    return fuse_stream(gates, n, width, [&](const qcs_gate& g) {
+        out.push_back(g);
+        return QCS_OK;
+    });
 }
 
 }  // namespace qcs
