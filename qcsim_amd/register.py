@@ -25,6 +25,12 @@ def _dptr(a: np.ndarray):
     return a.ctypes.data_as(_PD)
 
 
+# struct qcs_gate as a numpy record: the eight 32-bit header words (k, traits,
+# qubits[5], flags) and the interleaved matrix
+_GATE_DT = np.dtype([("head", np.uint32, 8), ("mat", np.float64, 2 * 32 * 32)])
+assert _GATE_DT.itemsize == C.sizeof(QcsGate)
+
+
 def _mat_buf(m: np.ndarray) -> np.ndarray:
     m = np.ascontiguousarray(m, dtype=np.complex128)
     return m.view(np.float64).reshape(-1)
@@ -206,17 +212,26 @@ class QubitRegister:
 
     @staticmethod
     def _gate_array(gates):
-        arr = (QcsGate * len(gates))()
+        """The gates as a struct qcs_gate array. The array is filled through a
+        numpy view of the same memory, one assignment per matrix size, and its
+        zero pages are left untouched past each matrix's 2*4^k doubles."""
+        n = len(gates)
+        a = np.zeros(n, dtype=_GATE_DT)
+        heads = []
+        mats = {}
         for i, (gate, qubits) in enumerate(gates):
-            m = _mat_buf(gate.getRawOperatorMatrix())
-            g = arr[i]
-            g.k = gate.getQubitsNumber()
-            g.traits = gate.traits
-            g.flags = getattr(gate, "flags", 0)
-            for j, q in enumerate(qubits):
-                g.qubits[j] = q
-            C.memmove(g.mat, m.ctypes.data, m.nbytes)
-        return arr
+            q = list(qubits) + [0] * (5 - len(qubits))
+            heads.append([gate.getQubitsNumber(), gate.traits, *q, getattr(gate, "flags", 0)])
+            m = gate.getRawOperatorMatrix()
+            rows, ms = mats.setdefault(m.size, ([], []))
+            rows.append(i)
+            ms.append(m)
+        if n:
+            a["head"] = np.asarray(heads, dtype=np.uint32)
+        for size, (rows, ms) in mats.items():
+            block = np.asarray(ms, dtype=np.complex128).reshape(len(rows), size)
+            a["mat"][rows, : 2 * size] = block.view(np.float64)
+        return (QcsGate * n).from_buffer(a)
 
     def apply_circuit(self, gates, fuse_width=0):
         """The gates in order; fuse_width 0 (none), 2..5 or "auto" (the width
