@@ -471,15 +471,14 @@ def qaoa_batch_instances(ninst, seed=31, sizes=(12, 14, 16, 18, 20, 22, 24)):
 
 def qaoa_eval_bytes(n, p, tile_bits, group):
     """HBM bytes of one evaluation of an n-qubit instance (complex64, 8 B per
-    amplitude) under the stage schedule: n <= tile bits: one LDS-resident
-    pass writing the final state; else per layer a tile pass (cost + Rx on
-    the low qubits; layer 0 writes |+> without reading) and ceil((n - T)/G)
-    register-group passes, the energy read fused into the last one."""
+    amplitude) and its passes over the state, from the library's own pass
+    plan (qcs_qaoa_plan): every pass reads and writes the state except the
+    first, which writes it from |+...+> (for n <= tile bits the only pass,
+    in LDS, writes the final state)."""
+    from qcsim_amd.qaoa import qaoa_plan
+
+    passes = len(qaoa_plan(n, p, tile_bits, group))
     A = 1 << n
-    if n <= tile_bits:
-        return 8 * A, 1
-    ng = -(-(n - tile_bits) // group)
-    passes = p * (1 + ng)
     return 16 * A * passes - 8 * A, passes
 
 

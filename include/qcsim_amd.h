@@ -265,6 +265,17 @@ int qcs_qaoa_last_ms(qcs_qaoa* q, float* ms);
 int qcs_qaoa_get_state(qcs_qaoa* q, uint32_t inst, double* amps);
 int qcs_qaoa_info(qcs_qaoa* q, uint32_t* stages, uint64_t* amplitudes,
                   uint32_t* tile_bits, uint32_t* group);
+/* The passes over the state that one evaluation of an n-qubit instance
+ * makes under (tile_bits, group) at depth p, in order (host only; no
+ * device or batch needed): 5 words per pass, (kind, width, mode, qlo,
+ * layer). kind 0: the whole evaluation in LDS (n <= tile_bits); 1: tile
+ * pass on qubits 0..tile_bits-1, mode 0 = cost diagonal of `layer` then its
+ * Rx, 1 = Rx only; 2: mixer pass on qubits qlo..qlo+width-1, mode 0 = Rx of
+ * `layer`, 1 = Rx then the energy, 2 = Rx of `layer`, the cost diagonal of
+ * layer + 1, Rx of layer + 1. Layer 0's first pass starts from |+...+>.
+ * Writes at most cap passes; *npasses receives the full count. */
+int qcs_qaoa_plan(uint32_t n, uint32_t tile_bits, uint32_t group, uint32_t p, uint32_t* out,
+                  uint32_t cap, uint32_t* npasses);
 
 /* ---- batched VQE energies (caller of the path: VQE.h:78-212) ----------- */
 

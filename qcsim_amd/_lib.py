@@ -90,6 +90,7 @@ SIGNATURES = {
     "qcs_qaoa_last_ms": (_I, [_P, C.POINTER(C.c_float)]),
     "qcs_qaoa_get_state": (_I, [_P, _U32, _PD]),
     "qcs_qaoa_info": (_I, [_P, _PU32, _PU64, _PU32, _PU32]),
+    "qcs_qaoa_plan": (_I, [_U32, _U32, _U32, _U32, _PU32, _U32, _PU32]),
     "qcs_vqe_create": (_I, [_U32, _PU32, _PU32, _PU64, _PU64, _PD, C.POINTER(_P)]),
     "qcs_vqe_destroy": (_I, [_P]),
     "qcs_vqe_evaluate": (_I, [_P, _PU32, _PD, _PD]),

@@ -16,15 +16,18 @@
 // Work decomposition for an n-qubit instance, tile bits T and group width G:
 //   n <= T: one SMALL unit runs every layer and the energy with the whole
 //           state in LDS (k_qaoa_tile<true>);
-//   else per layer
-//     tile launch (k_qaoa_tile<false>): the 2^(n-T) tiles of 2^T
-//           amplitudes, in units of up to 8 consecutive tiles, each tile in
-//           LDS: (layer 0: |+> generated in place) cost diagonal, Rx on
-//           qubits 0..T-1;
-//     mixer launches (k_qaoa_mix<g, E>) j = 0 .. ceil((n-T)/G) - 1: Rx on
-//           qubits T+jG .. (the last group may be narrower), each thread one
-//           2^g-amplitude orbit in registers; the last one of the last layer
-//           also accumulates the energy (E).
+//   else passes over the state of two kinds:
+//     tile pass (k_qaoa_tile<false>): the 2^(n-T) tiles of 2^T amplitudes,
+//           in units of up to 8 consecutive tiles, each tile in LDS:
+//           (layer 0: |+> generated in place) the cost diagonal, unless an
+//           earlier merged pass ran it, then Rx on qubits 0..T-1;
+//     mixer pass (k_qaoa_mix<g, MODE>) on group j of the ng = ceil((n-T)/G)
+//           groups of qubits T+jG .. (the last may be narrower), each thread
+//           one 2^g-amplitude orbit in registers: Rx; or Rx, the next
+//           layer's cost diagonal and its Rx (merged); or Rx and the energy
+//           (the last pass).
+//   The order of the passes is in instance_passes (p ng + 1 passes for
+//   ng >= 2, p (ng + 1) for ng = 1).
 // A launch pulls units from its own atomic ticket until none are left; the
 // launches of one evaluation are captured in a hipGraph. Instances of more
 // than 18 qubits are stored in a permuted order (arena_xor below).
@@ -60,7 +63,12 @@ struct QUnit {
     uint32_t qlo;    // first qubit of the mixer group
     uint32_t slot;   // energy partial slot (mixer with energy)
     uint32_t ntile;  // consecutive tiles from idx (tile launches)
+    uint32_t flags;  // tile launches: kTileRxOnly
 };
+
+// Tile unit flag: Rx(2 beta) on the tile qubits without the cost diagonal
+// (layer l's cost already ran in a merged mixer pass).
+constexpr uint32_t kTileRxOnly = 1;
 
 constexpr uint32_t kMaxSites = 64, kMaxEdges = 256;
 
@@ -436,11 +444,21 @@ __device__ void thread_terms(ThreadTerms& W, double gamma, const QInst& I, const
 // `tid` is threadIdx.x passed through an empty asm statement by the caller,
 // so that the compiler recomputes the LDS addresses per tile instead of
 // holding them all in registers across a run of tiles.
-template <int M>
+template <int M, bool COST>
 __device__ void tile_layer(float2* t, uint32_t tid, uint32_t jt, double gamma, const RxForm& rf, const QInst& I,
                            const IsingLds& P, TileCostLds& TC, const ThreadTerms& W, int in_mode,
                            const float2 (&xin)[1 << M], float amp) {
     constexpr uint32_t tb = 8 + M;
+    float2 y[1 << M];
+    if constexpr (!COST) {
+        // Rx alone: the orbit scaled by the rotations' real factor
+#pragma unroll
+        for (uint32_t m = 0; m < (1u << M); ++m) {
+            const float2 v = in_mode == IN_REG ? xin[m]
+                                               : (in_mode == IN_LDS ? t[swz(tid) + swz(m << 8)] : make_float2(amp, 0.f));
+            y[m] = make_float2(rf.scale * v.x, rf.scale * v.y);
+        }
+    } else {
     // per-thread terms, from the tile's terms (tile_terms, slot jt)
     double A = TC.K[jt] + W.c0;
 #pragma unroll
@@ -461,7 +479,6 @@ __device__ void tile_layer(float2* t, uint32_t tid, uint32_t jt, double gamma, c
     float2 ph = expmi(gamma * A);
     ph.x *= rf.scale;  // the real factor of the tb scaled rotations below
     ph.y *= rf.scale;
-    float2 y[1 << M];
 #pragma unroll
     for (uint32_t m = 0; m < (1u << M); ++m) {
         const uint32_t gm = m ^ (m >> 1);
@@ -477,6 +494,7 @@ __device__ void tile_layer(float2* t, uint32_t tid, uint32_t jt, double gamma, c
         const float2 v = in_mode == IN_REG ? xin[gm]
                                            : (in_mode == IN_LDS ? t[swz(tid) + swz(gm << 8)] : make_float2(amp, 0.f));
         y[gm] = make_float2(wr * v.x - wi * v.y, wr * v.y + wi * v.x);
+    }
     }
     // y is the orbit of qubits 8..tb-1: their Rx before the LDS store
     if constexpr (M > 0) rxs_reg<M>(y, rf);
@@ -513,14 +531,16 @@ __device__ void tile_rounds(float2* t, uint32_t tid, const RxForm& rf) {
 // is peeled off the loop, so that the loop head is reached with the same
 // memory operations outstanding on every path (the next tile's loads, then
 // this tile's stores) and waits only for the loads.
-template <int M>
+template <int M, bool COST>
 __device__ void tile_run(float2* t, float2* psi, uint64_t tile0, uint32_t ntile, double gamma, double c, double s,
                          const QInst& I, const IsingLds& P, TileCostLds& TC, bool first, float amp) {
     constexpr uint32_t tb = 8 + M, C = 1u << tb;
     const RxForm rf = rx_form(c, s, tb);
-    ThreadTerms W;
-    tile_terms<M>(TC, gamma, I, P, tile0, ntile);
-    thread_terms<M>(W, gamma, I, P, TC);
+    ThreadTerms W{};
+    if constexpr (COST) {
+        tile_terms<M>(TC, gamma, I, P, tile0, ntile);
+        thread_terms<M>(W, gamma, I, P, TC);
+    }
     // stored positions: element i of the tile at base is io[i ^ ci] (io =
     // psi + base for tiles of >= 1024 amplitudes; a smaller tile trades
     // elements with the tile at base ^ (the mask's bits >= tb), which has
@@ -545,7 +565,7 @@ __device__ void tile_run(float2* t, float2* psi, uint64_t tile0, uint32_t ntile,
         uint32_t tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
         __builtin_assume(tid < 256u);
-        tile_layer<M>(t, tid, j, gamma, rf, I, P, TC, W, first ? IN_PLUS : IN_REG, xin, amp);
+        tile_layer<M, COST>(t, tid, j, gamma, rf, I, P, TC, W, first ? IN_PLUS : IN_REG, xin, amp);
         if (!first && j + 1 < ntile) fetch(base + C);
         tile_rounds<M>(t, tid, rf);
         float2* const io = io_of(base);
@@ -585,21 +605,22 @@ __device__ void small_run(float2* t, const double* an, uint32_t p, const QInst& 
         __builtin_assume(tid < 256u);
         const RxForm rf = rx_form(cb, sb, 8 + M);
         float2 none[1 << M] = {};  // not read: the input is |+> or the LDS tile
-        tile_layer<M>(t, tid, 0, an[2 * l], rf, I, P, TC, W, l ? IN_LDS : IN_PLUS, none, amp);
+        tile_layer<M, true>(t, tid, 0, an[2 * l], rf, I, P, TC, W, l ? IN_LDS : IN_PLUS, none, amp);
         tile_rounds<M>(t, tid, rf);
     }
     __syncthreads();  // the energy pass reads every wave's elements
 }
 
-__device__ __forceinline__ void tile_run_tb(uint32_t tb, float2* t, float2* psi, uint64_t tile0, uint32_t ntile, double gamma,
-                            double c, double s, const QInst& I, const IsingLds& P, TileCostLds& TC, bool first,
-                            float amp) {
+template <bool COST>
+__device__ __forceinline__ void tile_run_tb(uint32_t tb, float2* t, float2* psi, uint64_t tile0, uint32_t ntile,
+                                            double gamma, double c, double s, const QInst& I, const IsingLds& P,
+                                            TileCostLds& TC, bool first, float amp) {
     switch (tb) {
-        case 8: tile_run<0>(t, psi, tile0, ntile, gamma, c, s, I, P, TC, first, amp); break;
-        case 9: tile_run<1>(t, psi, tile0, ntile, gamma, c, s, I, P, TC, first, amp); break;
-        case 10: tile_run<2>(t, psi, tile0, ntile, gamma, c, s, I, P, TC, first, amp); break;
-        case 11: tile_run<3>(t, psi, tile0, ntile, gamma, c, s, I, P, TC, first, amp); break;
-        default: tile_run<4>(t, psi, tile0, ntile, gamma, c, s, I, P, TC, first, amp); break;
+        case 8: tile_run<0, COST>(t, psi, tile0, ntile, gamma, c, s, I, P, TC, first, amp); break;
+        case 9: tile_run<1, COST>(t, psi, tile0, ntile, gamma, c, s, I, P, TC, first, amp); break;
+        case 10: tile_run<2, COST>(t, psi, tile0, ntile, gamma, c, s, I, P, TC, first, amp); break;
+        case 11: tile_run<3, COST>(t, psi, tile0, ntile, gamma, c, s, I, P, TC, first, amp); break;
+        default: tile_run<4, COST>(t, psi, tile0, ntile, gamma, c, s, I, P, TC, first, amp); break;
     }
 }
 
@@ -681,9 +702,14 @@ __global__ __launch_bounds__(256, 3) void k_qaoa_tile(float2* __restrict__ arena
             const float amp = float(1.0 / sqrt(double(1ull << I.n)));
             double sb, cb;
             sincos(an[2 * U.layer + 1], &sb, &cb);
+            const bool cost = !(U.flags & kTileRxOnly);
             if (tb >= 8) {
-                tile_run_tb(tb, tile, psi, U.idx, U.ntile, an[2 * U.layer], cb, sb, I, P, TC,
-                            U.layer == 0, amp);
+                if (cost)
+                    tile_run_tb<true>(tb, tile, psi, U.idx, U.ntile, an[2 * U.layer], cb, sb, I, P, TC,
+                                      U.layer == 0, amp);
+                else
+                    tile_run_tb<false>(tb, tile, psi, U.idx, U.ntile, an[2 * U.layer], cb, sb, I, P, TC,
+                                       U.layer == 0, amp);
             } else {
                 for (uint32_t j = 0; j < U.ntile; ++j) {
                     const uint64_t base = (uint64_t(U.idx) + j) << tb;
@@ -691,7 +717,7 @@ __global__ __launch_bounds__(256, 3) void k_qaoa_tile(float2* __restrict__ arena
                         tile[i] = U.layer ? ldamp<float, true>(psi + ((base + i) ^ arena_xor(base)))
                                           : make_float2(amp, 0.f);
                     __syncthreads();
-                    lds_cost(tile, tb, base, an[2 * U.layer], I, P);
+                    if (cost) lds_cost(tile, tb, base, an[2 * U.layer], I, P);
                     lds_rx(tile, tb, float(cb), float(sb));
                     for (uint32_t i = threadIdx.x; i < C; i += 256)
                         stamp<float, true>(psi + ((base + i) ^ arena_xor(base)), tile[i]);
@@ -703,16 +729,30 @@ __global__ __launch_bounds__(256, 3) void k_qaoa_tile(float2* __restrict__ arena
     }
 }
 
-// Mixer units: Rx(2 beta) on qubits qlo .. qlo+g-1 for kMixChunks chunks of
-// 256 orbits; with E, also this unit's share of the energy.
-template <int g, bool E>
+// Mixer modes: Rx(2 beta_l) alone; the same and this unit's share of the
+// energy (last pass of an evaluation); or a merged pass across the boundary
+// of layers l and l + 1: Rx(2 beta_l), the cost diagonal of layer l + 1,
+// Rx(2 beta_{l+1}), all on the group's orbit in registers. A merged pass is
+// valid when the group's Rx is the last of layer l's rotations to run (the
+// rotations of one layer act on different qubits and commute) and the first
+// of layer l + 1's; the cost diagonal, applied with every amplitude's full
+// index at hand, can run in any pass that sees each amplitude once.
+enum : int { MIX_PLAIN = 0, MIX_ENERGY = 1, MIX_MERGE = 2 };
+
+// Mixer units: kMixChunks chunks of 256 orbits of the qubits qlo ..
+// qlo+g-1, in the unit's mode.
+template <int g, int MODE>
 __global__ __launch_bounds__(256) void k_qaoa_mix(float2* __restrict__ arena, const QInst* __restrict__ insts,
                                                   const QUnit* __restrict__ units, uint32_t nunits,
                                                   uint32_t* __restrict__ ticket, const double* __restrict__ ang,
                                                   uint32_t p, IsingArgsDev ga, double* __restrict__ partial) {
     constexpr int D = 1 << g;
+    constexpr bool E = MODE == MIX_ENERGY, MG = MODE == MIX_MERGE;
+    // low part of the merged pass's two-level walk over the group bits
+    constexpr int GL = g < 3 ? g : 3, DL = 1 << GL, DH = D >> GL;
     __shared__ IsingLds P;
-    __shared__ double qe[D];  // E: -sum J z z over the edges inside the group, per group value
+    __shared__ double qe[E ? D : 1];   // E: -sum J z z over the edges inside the group, per group value
+    __shared__ float2 qph[MG ? D : 1];  // MG: exp(-i gamma' (Q(c) - Q(0))), Q = sum of J z z inside the group
     __shared__ double red[4];
     __shared__ uint32_t su;
     for (;;) {
@@ -723,8 +763,8 @@ __global__ __launch_bounds__(256) void k_qaoa_mix(float2* __restrict__ arena, co
         if (u >= nunits) break;
         const QUnit U = units[u];
         const QInst I = insts[U.inst];
+        if (E || MG) load_params(P, I, ga);
         if (E) {
-            load_params(P, I, ga);
             for (uint32_t m = threadIdx.x; m < uint32_t(D); m += 256) {
                 double v = 0;
                 for (uint32_t e = 0; e < I.ne; ++e) {
@@ -736,10 +776,27 @@ __global__ __launch_bounds__(256) void k_qaoa_mix(float2* __restrict__ arena, co
             }
             __syncthreads();
         }
+        const double* an = ang + size_t(U.inst) * p * 2;
+        // MG: gamma and beta of layer l + 1
+        const double gn = MG ? an[2 * (U.layer + 1)] : 0.0;
+        if (MG) {
+            for (uint32_t m = threadIdx.x; m < uint32_t(D); m += 256) {
+                double v = 0;  // Q(m) - Q(0): -2 J for each inside edge whose spins differ in m
+                for (uint32_t e = 0; e < I.ne; ++e) {
+                    const uint32_t a = P.ei[e], b = P.ej[e];
+                    if (a >= U.qlo && a < U.qlo + g && b >= U.qlo && b < U.qlo + g &&
+                        (((m >> (a - U.qlo)) ^ (m >> (b - U.qlo))) & 1))
+                        v -= 2.0 * P.J[e];
+                }
+                qph[m] = expmi(gn * v);
+            }
+            __syncthreads();
+        }
         float2* psi = arena + I.off;
-        double sb, cb;
-        sincos(ang[(size_t(U.inst) * p + U.layer) * 2 + 1], &sb, &cb);
-        const float fc = float(cb), fs = float(sb);
+        double sb, cb, sb2 = 0, cb2 = 1;
+        sincos(an[2 * U.layer + 1], &sb, &cb);
+        if (MG) sincos(an[2 * (U.layer + 1) + 1], &sb2, &cb2);
+        const float fc = float(cb), fs = float(sb), fc2 = float(cb2), fs2 = float(sb2);
         const uint64_t norb = (1ull << I.n) >> g;
         double acc = 0;
 #pragma unroll 1
@@ -757,11 +814,59 @@ __global__ __launch_bounds__(256) void k_qaoa_mix(float2* __restrict__ arena, co
             float2 x[D];
 #pragma unroll
             for (int c = 0; c < D; ++c) x[c] = ldamp<float, true>(psi + (pb ^ row(c)));
+            rx_reg<g>(x, fc, fs);
+            if (MG) {
+                // exp(-i gamma' C(base | c)) with z = 1 - 2 b:
+                //   C(base | c) = C(base) - 2 sum_{k: c_k = 1} f_k + Q(c) - Q(0),
+                // f_k = h_i / 2 + sum of J_ij z_j over the neighbours j of
+                // site i = qlo + k outside the group. The product over the
+                // set bits of c runs as a Gray walk over the high bits of c
+                // and, from each of its steps, one over the low GL bits
+                // (each walk multiplies by ek or its conjugate as the bit is
+                // set or cleared; at most 2^GL - 1 steps from a fresh start).
+                double hz, jzz;
+                ising_full(base, I.n, I.ne, P, hz, jzz);
+                float2 ek[g];
 #pragma unroll
-            for (int q = 0; q < g; ++q)
+                for (int k = 0; k < g; ++k) {
+                    const uint32_t i = U.qlo + k;
+                    double f = 0.5 * P.h[i];
+                    for (uint32_t a = P.aoff[i]; a < P.aoff[i + 1]; ++a) {
+                        const uint32_t j = P.aj[a];
+                        if (j < U.qlo || j >= U.qlo + g) f += ((base >> j) & 1) ? -adj_J(P, a) : adj_J(P, a);
+                    }
+                    ek[k] = expmi(-2.0 * gn * f);  // exp(2 i gamma' f_k)
+                }
+                auto step = [&](float2 ph, int k, bool set) {
+                    const float2 f = ek[k];
+                    const float fy = set ? f.y : -f.y;
+                    return make_float2(ph.x * f.x - ph.y * fy, ph.x * fy + ph.y * f.x);
+                };
+                float2 pho = expmi(gn * cost_of(hz, jzz));
 #pragma unroll
-                for (int c = 0; c < D; ++c)
-                    if (!((c >> q) & 1)) rx_pair(x[c], x[c | (1 << q)], fc, fs);
+                for (int mh = 0; mh < DH; ++mh) {
+                    const int gh = mh ^ (mh >> 1);
+                    if (mh) {
+                        const int k = __builtin_ctz(uint32_t(mh));
+                        pho = step(pho, GL + k, (gh >> k) & 1);
+                    }
+                    float2 ph = pho;
+#pragma unroll
+                    for (int ml = 0; ml < DL; ++ml) {
+                        const int gl = ml ^ (ml >> 1);
+                        if (ml) {
+                            const int k = __builtin_ctz(uint32_t(ml));
+                            ph = step(ph, k, (gl >> k) & 1);
+                        }
+                        const int c = (gh << GL) | gl;
+                        const float2 qm = qph[c];
+                        const float wr = ph.x * qm.x - ph.y * qm.y, wi = ph.x * qm.y + ph.y * qm.x;
+                        const float2 v = x[c];
+                        x[c] = make_float2(wr * v.x - wi * v.y, wr * v.y + wi * v.x);
+                    }
+                }
+                rx_reg<g>(x, fc2, fs2);
+            }
             if (E) {
                 // E over the orbit: E(base) in full, then along a Gray walk
                 // over the group bits: setting bit k of the group (spin
@@ -849,7 +954,7 @@ struct qcs_qaoa {
     // launch plan: one persistent launch per entry, in order
     enum : uint8_t { L_SMALL = 0, L_TILE = 1, L_MIX = 2 };
     struct Launch {
-        uint8_t kind, g, energy;
+        uint8_t kind, g, mode;  // mode: MIX_* for mixer launches
         uint32_t nunits;
         QUnit* units;
     };
@@ -907,14 +1012,14 @@ static int launch_sequence(qcs_qaoa* q) {
                                    L.units, L.nunits, tk, q->d_ang, q->p, ga, q->d_energy);
             continue;
         }
-#define QCS_MIX(GG)                                                                                            \
-    do {                                                                                                       \
-        if (L.energy)                                                                                          \
-            hipLaunchKernelGGL((k_qaoa_mix<GG, true>), dim3(grid), dim3(256), 0, q->stream, q->arena, q->d_inst, \
-                               L.units, L.nunits, tk, q->d_ang, q->p, ga, q->d_partial);                      \
-        else                                                                                                   \
-            hipLaunchKernelGGL((k_qaoa_mix<GG, false>), dim3(grid), dim3(256), 0, q->stream, q->arena,        \
-                               q->d_inst, L.units, L.nunits, tk, q->d_ang, q->p, ga, q->d_partial);           \
+#define QCS_MIX_MODE(GG, MM)                                                                                    \
+    hipLaunchKernelGGL((k_qaoa_mix<GG, MM>), dim3(grid), dim3(256), 0, q->stream, q->arena, q->d_inst, L.units, \
+                       L.nunits, tk, q->d_ang, q->p, ga, q->d_partial)
+#define QCS_MIX(GG)                                                       \
+    do {                                                                  \
+        if (L.mode == MIX_ENERGY) QCS_MIX_MODE(GG, MIX_ENERGY);           \
+        else if (L.mode == MIX_MERGE) QCS_MIX_MODE(GG, MIX_MERGE);        \
+        else QCS_MIX_MODE(GG, MIX_PLAIN);                                 \
     } while (0)
         switch (L.g) {
             case 1: QCS_MIX(1); break;
@@ -925,6 +1030,7 @@ static int launch_sequence(qcs_qaoa* q) {
             default: QCS_MIX(6); break;
         }
 #undef QCS_MIX
+#undef QCS_MIX_MODE
     }
     if (q->nlarge)
         hipLaunchKernelGGL(k_qaoa_energy, dim3(std::min<uint32_t>(q->nlarge, 1024)), dim3(256), 0, q->stream,
@@ -933,12 +1039,49 @@ static int launch_sequence(qcs_qaoa* q) {
     return e == hipSuccess ? QCS_OK : set_error(QCS_EDEVICE, "qaoa launch failed: %s", hipGetErrorString(e));
 }
 
+// One pass of a large instance over its amplitudes.
+struct PassSpec {
+    uint8_t kind;   // qcs_qaoa::L_TILE or L_MIX
+    uint8_t g;      // mixer group width
+    uint8_t mode;   // mixer: MIX_*; tile: 0 or kTileRxOnly
+    uint32_t qlo;   // mixer: first qubit of the group
+    uint32_t layer;
+};
+
+// The passes of one evaluation of an instance of n > T qubits whose qubits
+// above the tile form ng mixer groups. ng = 1: per layer the tile pass (cost
+// diagonal, Rx on the tile qubits) and the group's Rx. ng >= 2: the last two
+// groups take turns at ending a layer with a merged pass (Rx of layer l,
+// cost of layer l + 1, Rx of layer l + 1); the next layer's tile pass then
+// runs without the cost and that layer skips the merged group. That is
+// p ng + 1 passes instead of p (ng + 1): 9 instead of 12 for two groups at
+// depth 4. Every layer's cost runs before its rotations, and every rotation
+// of a layer before the next layer's cost.
+static std::vector<PassSpec> instance_passes(uint32_t n, uint32_t Tm, uint32_t G, uint32_t p) {
+    const uint32_t ng = (n - Tm + G - 1) / G;
+    auto width = [&](uint32_t j) { return uint8_t(j == ng - 1 ? (n - Tm) - G * (ng - 1) : G); };
+    std::vector<PassSpec> v;
+    int merged = -1;  // the group whose Rx of this layer ran in the previous layer's merged pass
+    for (uint32_t l = 0; l < p; ++l) {
+        v.push_back(PassSpec{qcs_qaoa::L_TILE, 0, uint8_t(merged >= 0 ? kTileRxOnly : 0), 0, l});
+        const int z = ng >= 2 ? int(l % 2 == 0 ? ng - 1 : ng - 2) : int(ng - 1);
+        for (uint32_t j = 0; j < ng; ++j)
+            if (int(j) != merged && int(j) != z)
+                v.push_back(PassSpec{qcs_qaoa::L_MIX, width(j), MIX_PLAIN, Tm + j * G, l});
+        const bool last = l + 1 == p;
+        const uint8_t zm = last ? MIX_ENERGY : (ng >= 2 ? MIX_MERGE : MIX_PLAIN);
+        v.push_back(PassSpec{qcs_qaoa::L_MIX, width(uint32_t(z)), zm, Tm + uint32_t(z) * G, l});
+        merged = (!last && ng >= 2) ? z : -1;
+    }
+    return v;
+}
+
 // Build the launch plan for (tmax, group) and capture it.
 static int build_plan(qcs_qaoa* q) {
     free_plan(q);
     const uint32_t G = q->group, Tm = q->tmax;
     struct Pending {
-        uint8_t kind, g, energy;
+        uint8_t kind, g, mode;
         std::vector<QUnit> units;
     };
     std::vector<Pending> pend;
@@ -952,47 +1095,47 @@ static int build_plan(qcs_qaoa* q) {
             I.pslot0 = 0;
             I.npslots = 0;
             if (pend.empty()) pend.push_back(Pending{qcs_qaoa::L_SMALL, 0, 0, {}});
-            pend.back().units.push_back(QUnit{i, 0, 0, 0, 0, 0});
+            pend.back().units.push_back(QUnit{i, 0, 0, 0, 0, 0, 0});
         }
     }
-    uint32_t maxgroups = 0;
     for (uint32_t i = 0; i < q->ninst; ++i)
-        if (q->nq[i] > Tm) {
-            maxgroups = std::max(maxgroups, (q->nq[i] - Tm + G - 1) / G);
-            large.push_back(i);
-        }
-    // energy slots: one per unit of each instance's last mixer launch
+        if (q->nq[i] > Tm) large.push_back(i);
+    std::vector<std::vector<PassSpec>> passes;
+    size_t nstep = 0;
     for (uint32_t i : large) {
+        passes.push_back(instance_passes(q->nq[i], Tm, G, q->p));
+        nstep = std::max(nstep, passes.back().size());
+        // energy slots: one per unit of the instance's last pass
         QInst& I = q->inst_h[i];
-        const uint32_t n = q->nq[i], ng = (n - Tm + G - 1) / G;
-        const uint32_t glast = (n - Tm) - G * (ng - 1);
-        const uint64_t chunks = (((1ull << n) >> glast) + 255) / 256;
+        const uint64_t chunks = (((1ull << q->nq[i]) >> passes.back().back().g) + 255) / 256;
         I.tbits = Tm;
         I.pslot0 = pslot;
         I.npslots = uint32_t((chunks + kMixChunks - 1) / kMixChunks);
         pslot += I.npslots;
     }
-    for (uint32_t l = 0; l < q->p && !large.empty(); ++l) {
-        pend.push_back(Pending{qcs_qaoa::L_TILE, 0, 0, {}});
-        for (uint32_t i : large)
-            for (uint32_t t = 0, nt = 1u << (q->nq[i] - Tm); t < nt; t += kTileRun)
-                pend.back().units.push_back(QUnit{i, t, l, 0, 0, std::min(kTileRun, nt - t)});
-        for (uint32_t gi = 0; gi < maxgroups; ++gi) {
-            // one launch per (width, energy) present in this step
-            const size_t first = pend.size();
-            for (uint32_t i : large) {
-                const uint32_t n = q->nq[i], ng = (n - Tm + G - 1) / G;
-                if (gi >= ng) continue;
-                const uint32_t gw = gi == ng - 1 ? (n - Tm) - G * (ng - 1) : G;
-                const uint8_t en = (l == q->p - 1) && (gi == ng - 1);
-                size_t k = first;
-                while (k < pend.size() && !(pend[k].g == gw && pend[k].energy == en)) ++k;
-                if (k == pend.size()) pend.push_back(Pending{qcs_qaoa::L_MIX, uint8_t(gw), en, {}});
-                auto& P = pend[k];
-                const uint64_t chunks = (((1ull << n) >> gw) + 255) / 256;
+    // step s: pass s of every instance that has one, one launch per
+    // (kind, width, mode) present
+    for (size_t s = 0; s < nstep; ++s) {
+        const size_t first = pend.size();
+        for (size_t k = 0; k < large.size(); ++k) {
+            if (s >= passes[k].size()) continue;
+            const PassSpec& S = passes[k][s];
+            const uint32_t i = large[k], n = q->nq[i];
+            const bool tile = S.kind == qcs_qaoa::L_TILE;
+            const uint8_t g = tile ? 0 : S.g, md = tile ? 0 : S.mode;
+            size_t j = first;
+            while (j < pend.size() && !(pend[j].kind == S.kind && pend[j].g == g && pend[j].mode == md)) ++j;
+            if (j == pend.size()) pend.push_back(Pending{S.kind, g, md, {}});
+            auto& P = pend[j];
+            if (tile) {
+                for (uint32_t t = 0, nt = 1u << (n - Tm); t < nt; t += kTileRun)
+                    P.units.push_back(QUnit{i, t, S.layer, 0, 0, std::min(kTileRun, nt - t), S.mode});
+            } else {
+                const bool en = S.mode == MIX_ENERGY;
+                const uint64_t chunks = (((1ull << n) >> S.g) + 255) / 256;
                 uint32_t slot = q->inst_h[i].pslot0;
                 for (uint64_t c = 0; c < chunks; c += kMixChunks)
-                    P.units.push_back(QUnit{i, uint32_t(c), l, Tm + gi * G, en ? slot++ : 0u, 0});
+                    P.units.push_back(QUnit{i, uint32_t(c), S.layer, S.qlo, en ? slot++ : 0u, 0, 0});
             }
         }
     }
@@ -1004,7 +1147,7 @@ static int build_plan(qcs_qaoa* q) {
         QUnit* d = nullptr;
         if (hipMalloc(reinterpret_cast<void**>(&d), sizeof(QUnit) * P.units.size()) != hipSuccess)
             return set_error(QCS_ENOMEM, "unit table");
-        q->plan.push_back(qcs_qaoa::Launch{P.kind, P.g, P.energy, uint32_t(P.units.size()), d});
+        q->plan.push_back(qcs_qaoa::Launch{P.kind, P.g, P.mode, uint32_t(P.units.size()), d});
         if (hipMemcpy(d, P.units.data(), sizeof(QUnit) * P.units.size(), hipMemcpyHostToDevice) != hipSuccess)
             return set_error(QCS_EDEVICE, "unit upload failed");
     }
@@ -1221,6 +1364,29 @@ int qcs_qaoa_get_state(qcs_qaoa* q, uint32_t inst, double* amps) {
         amps[2 * i] = v.x;
         amps[2 * i + 1] = v.y;
     }
+    return QCS_OK;
+}
+
+int qcs_qaoa_plan(uint32_t n, uint32_t tile_bits, uint32_t group, uint32_t p, uint32_t* out, uint32_t cap,
+                  uint32_t* npasses) {
+    if (!npasses || (cap && !out) || n == 0 || n > 40 || p == 0) return set_error(QCS_EINVAL, "bad argument");
+    if (tile_bits < 6 || tile_bits > 12) return set_error(QCS_EINVAL, "tile bits must be 6..12");
+    if (group < 4 || group > 6) return set_error(QCS_EINVAL, "group width must be 4..6");
+    if (n <= tile_bits) {
+        if (cap) {
+            const uint32_t w[5] = {0, 0, 0, 0, 0};
+            std::memcpy(out, w, sizeof w);
+        }
+        *npasses = 1;
+        return QCS_OK;
+    }
+    const std::vector<PassSpec> v = instance_passes(n, tile_bits, group, p);
+    for (size_t k = 0; k < v.size() && k < cap; ++k) {
+        const PassSpec& S = v[k];
+        const uint32_t w[5] = {S.kind == qcs_qaoa::L_TILE ? 1u : 2u, S.g, S.mode, S.qlo, S.layer};
+        std::memcpy(out + 5 * k, w, sizeof w);
+    }
+    *npasses = uint32_t(v.size());
     return QCS_OK;
 }
 

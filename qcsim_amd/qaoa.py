@@ -75,6 +75,24 @@ def random_ising(n: int, rng: np.random.Generator, j_sigma=1.0, h_sigma=0.5) -> 
     return IsingInstance(n, edges, J, h)
 
 
+PLAN_KINDS = {0: "lds", 1: "tile", 2: "mixer"}
+
+
+def qaoa_plan(n: int, p: int, tile_bits: int = 12, group: int = 6):
+    """The passes over the state that one evaluation of an n-qubit instance
+    makes (qcs_qaoa_plan; host only), in order, as (kind, width, mode, qlo,
+    layer) with kind "lds" (the whole evaluation in LDS), "tile" (mode 0:
+    cost then Rx of `layer` on qubits 0..tile_bits-1; 1: Rx only) or "mixer"
+    (qubits qlo..qlo+width-1; mode 0: Rx; 1: Rx then the energy; 2: Rx of
+    `layer`, cost of layer + 1, Rx of layer + 1)."""
+    cnt = C.c_uint32()
+    check(lib().qcs_qaoa_plan(n, tile_bits, group, p, None, 0, C.byref(cnt)))
+    buf = (C.c_uint32 * (5 * cnt.value))()
+    check(lib().qcs_qaoa_plan(n, tile_bits, group, p, buf, cnt.value, C.byref(cnt)))
+    w = list(buf)
+    return [(PLAN_KINDS[w[5 * k]], *w[5 * k + 1 : 5 * k + 5]) for k in range(cnt.value)]
+
+
 class QaoaBatch:
     """Many instances evaluated together on the GPU (qcs_qaoa_*)."""
 

@@ -51,14 +51,15 @@ def test_workload_generators_are_deterministic():
 
 @pytest.mark.parametrize("n,p,tb,grp", [(12, 4, 12, 5), (24, 4, 12, 6), (24, 4, 12, 5), (18, 3, 10, 4)])
 def test_qaoa_bytes_follow_the_pass_plan(n, p, tb, grp):
+    from qcsim_amd.qaoa import qaoa_plan
+
+    plan = qaoa_plan(n, p, tb, grp)
     amp = 8 * (1 << n)
-    if n <= tb:
-        expect, passes = amp, 1  # one LDS-resident pass writing the final state
-    else:
-        passes = p * (1 + -(-(n - tb) // grp))
-        expect = 2 * amp * passes - amp  # layer 0's tile pass writes without reading
-    assert bench.qaoa_eval_bytes(n, p, tb, grp) == (expect, passes)
+    # the first pass writes without reading, every later one reads and writes
+    assert bench.qaoa_eval_bytes(n, p, tb, grp) == (2 * amp * len(plan) - amp, len(plan))
     if (n, tb, grp) == (24, 12, 6):
-        assert passes == 12  # two width-6 mixer passes per layer
+        assert len(plan) == 9  # 3 passes in layer 0, 2 in each later one
     if (n, tb, grp) == (24, 12, 5):
-        assert passes == 16  # 5 + 5 + 2
+        assert len(plan) == 13  # groups 5 + 5 + 2: 4 in layer 0, 3 later
+    if (n, tb, grp) == (18, 10, 4):
+        assert len(plan) == 7  # groups 4 + 4
