@@ -814,35 +814,49 @@ __global__ __launch_bounds__(256) void k_qaoa_mix(float2* __restrict__ arena, co
             float2 x[D];
 #pragma unroll
             for (int c = 0; c < D; ++c) x[c] = ldamp<float, true>(psi + (pb ^ row(c)));
-            rx_reg<g>(x, fc, fs);
-            if (MG) {
-                // exp(-i gamma' C(base | c)) with z = 1 - 2 b:
-                //   C(base | c) = C(base) - 2 sum_{k: c_k = 1} f_k + Q(c) - Q(0),
-                // f_k = h_i / 2 + sum of J_ij z_j over the neighbours j of
-                // site i = qlo + k outside the group. The product over the
-                // set bits of c runs as a Gray walk over the high bits of c
-                // and, from each of its steps, one over the low GL bits
-                // (each walk multiplies by ek or its conjugate as the bit is
-                // set or cleared; at most 2^GL - 1 steps from a fresh start).
+            // Terms that depend on the orbit's fixed spins alone, computed
+            // while the loads are in flight. With z = 1 - 2 b, site
+            // i = qlo + k and n_k = sum of J_ij z_j over the neighbours j of
+            // i outside the group, setting bit k of c (z_i: +1 -> -1)
+            //   MG: changes C = sum J z z + sum h z / 2 by -2 f_k,
+            //       fk[k] = f_k = h_i / 2 + n_k, and the inside edges by
+            //       Q(c) - Q(0), so C(base | c) = C(base) - 2 sum_{k: c_k = 1}
+            //       f_k + Q(c) - Q(0);
+            //   E:  changes E = -(sum h z + sum J z z) by fk[k] = 2 (h_i + n_k)
+            //       outside the group; E(base | c) = E(base) - qe[0] +
+            //       sum_{k: c_k = 1} fk[k] + qe[c].
+            double tb0 = 0, fk[g];
+            if (MG || E) {
                 double hz, jzz;
                 ising_full(base, I.n, I.ne, P, hz, jzz);
-                float2 ek[g];
+                tb0 = MG ? cost_of(hz, jzz) : energy_of(hz, jzz) - qe[0];
 #pragma unroll
                 for (int k = 0; k < g; ++k) {
                     const uint32_t i = U.qlo + k;
-                    double f = 0.5 * P.h[i];
+                    double nk = 0;
                     for (uint32_t a = P.aoff[i]; a < P.aoff[i + 1]; ++a) {
                         const uint32_t j = P.aj[a];
-                        if (j < U.qlo || j >= U.qlo + g) f += ((base >> j) & 1) ? -adj_J(P, a) : adj_J(P, a);
+                        if (j < U.qlo || j >= U.qlo + g) nk += ((base >> j) & 1) ? -adj_J(P, a) : adj_J(P, a);
                     }
-                    ek[k] = expmi(-2.0 * gn * f);  // exp(2 i gamma' f_k)
+                    fk[k] = MG ? 0.5 * P.h[i] + nk : 2.0 * (P.h[i] + nk);
                 }
+            }
+            rx_reg<g>(x, fc, fs);
+            if (MG) {
+                // exp(-i gamma' C(base | c)): the product over the set bits
+                // of c runs as a Gray walk over the high bits of c and, from
+                // each of its steps, one over the low GL bits (each walk
+                // multiplies by ek or its conjugate as the bit is set or
+                // cleared; at most 2^GL - 1 steps from a fresh start).
+                float2 ek[g];
+#pragma unroll
+                for (int k = 0; k < g; ++k) ek[k] = expmi(-2.0 * gn * fk[k]);  // exp(2 i gamma' f_k)
                 auto step = [&](float2 ph, int k, bool set) {
                     const float2 f = ek[k];
                     const float fy = set ? f.y : -f.y;
                     return make_float2(ph.x * f.x - ph.y * fy, ph.x * fy + ph.y * f.x);
                 };
-                float2 pho = expmi(gn * cost_of(hz, jzz));
+                float2 pho = expmi(gn * tb0);
 #pragma unroll
                 for (int mh = 0; mh < DH; ++mh) {
                     const int gh = mh ^ (mh >> 1);
@@ -868,34 +882,22 @@ __global__ __launch_bounds__(256) void k_qaoa_mix(float2* __restrict__ arena, co
                 rx_reg<g>(x, fc2, fs2);
             }
             if (E) {
-                // E over the orbit: E(base) in full, then along a Gray walk
-                // over the group bits: setting bit k of the group (spin
-                // i = qlo + k) adds 2 (h_i + sum of J_ij z_j over neighbours
-                // j outside the group), clearing it subtracts that; the
-                // edges inside the group come from the table qe.
-                double hz, jzz;
-                ising_full(base, I.n, I.ne, P, hz, jzz);
-                const double e0 = energy_of(hz, jzz) - qe[0];
-                double g2[g];
-#pragma unroll
-                for (int k = 0; k < g; ++k) {
-                    const uint32_t i = U.qlo + k;
-                    double f = P.h[i];
-                    for (uint32_t a = P.aoff[i]; a < P.aoff[i + 1]; ++a) {
-                        const uint32_t j = P.aj[a];
-                        if (j < U.qlo || j >= U.qlo + g) f += ((base >> j) & 1) ? -adj_J(P, a) : adj_J(P, a);
-                    }
-                    g2[k] = 2.0 * f;
-                }
+                // sum over the orbit of |x_c|^2 E(base | c) along a Gray walk
+                // over c (setting bit k adds fk[k], clearing subtracts it).
+                // The table index passes through an opaque zero every 8
+                // steps, so that the compiler cannot hoist all 2^g table
+                // reads (2 VGPRs each) above the walk.
                 double dsum = 0;
+                uint32_t zq = 0;
 #pragma unroll
                 for (int m = 0; m < D; ++m) {
                     const int c = m ^ (m >> 1);
                     if (m) {
                         const int k = __builtin_ctz(uint32_t(m));
-                        dsum += ((c >> k) & 1) ? g2[k] : -g2[k];
+                        dsum += ((c >> k) & 1) ? fk[k] : -fk[k];
                     }
-                    acc += (double(x[c].x) * x[c].x + double(x[c].y) * x[c].y) * (e0 + dsum + qe[c]);
+                    if ((m & 7) == 0) asm volatile("" : "+v"(zq));
+                    acc += (double(x[c].x) * x[c].x + double(x[c].y) * x[c].y) * (tb0 + dsum + qe[uint32_t(c) | zq]);
                 }
             }
             // The store addresses are recomputed from a copy of `base` the
