@@ -272,7 +272,7 @@ int qcs_qaoa_info(qcs_qaoa* q, uint32_t* stages, uint64_t* amplitudes,
  * pass on qubits 0..tile_bits-1, mode 0 = cost diagonal of `layer` then its
  * Rx, 1 = Rx only; 2: mixer pass on qubits qlo..qlo+width-1, mode 0 = Rx of
  * `layer`, 1 = Rx then the energy, 2 = Rx of `layer`, the cost diagonal of
- * layer + 1, Rx of layer + 1. Layer 0's first pass starts from |+...+>.
+ * layer + 1, Rx of layer + 1. The first pass starts from |+...+>.
  * Writes at most cap passes; *npasses receives the full count. */
 int qcs_qaoa_plan(uint32_t n, uint32_t tile_bits, uint32_t group, uint32_t p, uint32_t* out,
                   uint32_t cap, uint32_t* npasses);

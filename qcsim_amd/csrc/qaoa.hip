@@ -18,9 +18,9 @@
 //           state in LDS (k_qaoa_tile<true>);
 //   else passes over the state of two kinds:
 //     tile pass (k_qaoa_tile<false>): the 2^(n-T) tiles of 2^T amplitudes,
-//           in units of up to 8 consecutive tiles, each tile in LDS:
-//           (layer 0: |+> generated in place) the cost diagonal, unless an
-//           earlier merged pass ran it, then Rx on qubits 0..T-1;
+//           in units of consecutive tiles, each tile in LDS: the cost
+//           diagonal (on |+> generated in place in layer 0) unless a mixer
+//           pass ran it, then Rx on qubits 0..T-1;
 //     mixer pass (k_qaoa_mix<g, MODE>) on group j of the ng = ceil((n-T)/G)
 //           groups of qubits T+jG .. (the last may be narrower), each thread
 //           one 2^g-amplitude orbit in registers: Rx; or Rx, the next
@@ -67,7 +67,7 @@ struct QUnit {
 };
 
 // Tile unit flag: Rx(2 beta) on the tile qubits without the cost diagonal
-// (layer l's cost already ran in a merged mixer pass).
+// (the layer's cost already ran in a merged mixer pass).
 constexpr uint32_t kTileRxOnly = 1;
 
 constexpr uint32_t kMaxSites = 64, kMaxEdges = 256;
@@ -84,7 +84,8 @@ __host__ __device__ __forceinline__ uint64_t arena_xor(uint64_t i) {
     return (((i >> 18) ^ (i >> 24)) & 63u) << 4;
 }
 constexpr uint32_t kMixChunks = 4;  // 256-orbit chunks per mixer unit
-constexpr uint32_t kTileRun = 8;    // tiles per tile unit (one instance)
+constexpr uint32_t kTileRun = 8;    // tiles per tile unit (one instance) with the cost diagonal
+constexpr uint32_t kTileRunRx = 32; // tiles per Rx-only tile unit (no per-tile terms to stage)
 
 // Instance parameters staged in LDS by every unit.
 struct IsingLds {
@@ -446,9 +447,8 @@ __device__ void thread_terms(ThreadTerms& W, double gamma, const QInst& I, const
 // holding them all in registers across a run of tiles.
 template <int M, bool COST>
 __device__ void tile_layer(float2* t, uint32_t tid, uint32_t jt, double gamma, const RxForm& rf, const QInst& I,
-                           const IsingLds& P, TileCostLds& TC, const ThreadTerms& W, int in_mode,
+                           const TileCostLds* TC, const ThreadTerms& W, int in_mode,
                            const float2 (&xin)[1 << M], float amp) {
-    constexpr uint32_t tb = 8 + M;
     float2 y[1 << M];
     if constexpr (!COST) {
         // Rx alone: the orbit scaled by the rotations' real factor
@@ -460,18 +460,18 @@ __device__ void tile_layer(float2* t, uint32_t tid, uint32_t jt, double gamma, c
         }
     } else {
     // per-thread terms, from the tile's terms (tile_terms, slot jt)
-    double A = TC.K[jt] + W.c0;
+    double A = TC->K[jt] + W.c0;
 #pragma unroll
-    for (uint32_t i = 0; i < 8; ++i) A += zf(tid, i) * TC.F[jt][i];
+    for (uint32_t i = 0; i < 8; ++i) A += zf(tid, i) * TC->F[jt][i];
     float2 e[M > 0 ? M : 1];
 #pragma unroll
     for (int k = 0; k < M; ++k) {
-        A += TC.F[jt][8 + k];
+        A += TC->F[jt][8 + k];
         // exp(2 i gamma F'_k) = ef eg, multiplied and rescaled to modulus 1
         // (to O(delta^2) for |.|^2 = 1 + delta) in FP64, then rounded, so
         // that the walk's alternating products by it and its conjugate cancel
         // to FP32 rounding
-        const float2 a = TC.ef[jt][k], b = W.eg[k];
+        const float2 a = TC->ef[jt][k], b = W.eg[k];
         const double x = double(a.x) * b.x - double(a.y) * b.y, y = double(a.x) * b.y + double(a.y) * b.x;
         const double r = 1.5 - 0.5 * (x * x + y * y);
         e[k] = make_float2(float(x * r), float(y * r));
@@ -488,7 +488,7 @@ __device__ void tile_layer(float2* t, uint32_t tid, uint32_t jt, double gamma, c
             const float fy = ((gm >> k) & 1) ? f.y : -f.y;  // set: times e_k; cleared: divided by it
             ph = make_float2(ph.x * f.x - ph.y * fy, ph.x * fy + ph.y * f.x);
         }
-        const float2 qm = TC.q[gm];
+        const float2 qm = TC->q[gm];
         const float pr = ph.x, pi = ph.y;
         const float wr = pr * qm.x - pi * qm.y, wi = pr * qm.y + pi * qm.x;
         const float2 v = in_mode == IN_REG ? xin[gm]
@@ -533,13 +533,13 @@ __device__ void tile_rounds(float2* t, uint32_t tid, const RxForm& rf) {
 // this tile's stores) and waits only for the loads.
 template <int M, bool COST>
 __device__ void tile_run(float2* t, float2* psi, uint64_t tile0, uint32_t ntile, double gamma, double c, double s,
-                         const QInst& I, const IsingLds& P, TileCostLds& TC, bool first, float amp) {
+                         const QInst& I, const IsingLds* P, TileCostLds* TC, bool first, float amp) {
     constexpr uint32_t tb = 8 + M, C = 1u << tb;
     const RxForm rf = rx_form(c, s, tb);
     ThreadTerms W{};
     if constexpr (COST) {
-        tile_terms<M>(TC, gamma, I, P, tile0, ntile);
-        thread_terms<M>(W, gamma, I, P, TC);
+        tile_terms<M>(*TC, gamma, I, *P, tile0, ntile);
+        thread_terms<M>(W, gamma, I, *P, *TC);
     }
     // stored positions: element i of the tile at base is io[i ^ ci] (io =
     // psi + base for tiles of >= 1024 amplitudes; a smaller tile trades
@@ -565,7 +565,7 @@ __device__ void tile_run(float2* t, float2* psi, uint64_t tile0, uint32_t ntile,
         uint32_t tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
         __builtin_assume(tid < 256u);
-        tile_layer<M, COST>(t, tid, j, gamma, rf, I, P, TC, W, first ? IN_PLUS : IN_REG, xin, amp);
+        tile_layer<M, COST>(t, tid, j, gamma, rf, I, TC, W, first ? IN_PLUS : IN_REG, xin, amp);
         if (!first && j + 1 < ntile) fetch(base + C);
         tile_rounds<M>(t, tid, rf);
         float2* const io = io_of(base);
@@ -605,7 +605,7 @@ __device__ void small_run(float2* t, const double* an, uint32_t p, const QInst& 
         __builtin_assume(tid < 256u);
         const RxForm rf = rx_form(cb, sb, 8 + M);
         float2 none[1 << M] = {};  // not read: the input is |+> or the LDS tile
-        tile_layer<M, true>(t, tid, 0, an[2 * l], rf, I, P, TC, W, l ? IN_LDS : IN_PLUS, none, amp);
+        tile_layer<M, true>(t, tid, 0, an[2 * l], rf, I, &TC, W, l ? IN_LDS : IN_PLUS, none, amp);
         tile_rounds<M>(t, tid, rf);
     }
     __syncthreads();  // the energy pass reads every wave's elements
@@ -613,8 +613,8 @@ __device__ void small_run(float2* t, const double* an, uint32_t p, const QInst& 
 
 template <bool COST>
 __device__ __forceinline__ void tile_run_tb(uint32_t tb, float2* t, float2* psi, uint64_t tile0, uint32_t ntile,
-                                            double gamma, double c, double s, const QInst& I, const IsingLds& P,
-                                            TileCostLds& TC, bool first, float amp) {
+                                            double gamma, double c, double s, const QInst& I, const IsingLds* P,
+                                            TileCostLds* TC, bool first, float amp) {
     switch (tb) {
         case 8: tile_run<0, COST>(t, psi, tile0, ntile, gamma, c, s, I, P, TC, first, amp); break;
         case 9: tile_run<1, COST>(t, psi, tile0, ntile, gamma, c, s, I, P, TC, first, amp); break;
@@ -702,27 +702,65 @@ __global__ __launch_bounds__(256, 3) void k_qaoa_tile(float2* __restrict__ arena
             const float amp = float(1.0 / sqrt(double(1ull << I.n)));
             double sb, cb;
             sincos(an[2 * U.layer + 1], &sb, &cb);
-            const bool cost = !(U.flags & kTileRxOnly);
+            // the first pass of an evaluation generates |+...+> in place
+            const bool gen = U.layer == 0;
             if (tb >= 8) {
-                if (cost)
-                    tile_run_tb<true>(tb, tile, psi, U.idx, U.ntile, an[2 * U.layer], cb, sb, I, P, TC,
-                                      U.layer == 0, amp);
-                else
-                    tile_run_tb<false>(tb, tile, psi, U.idx, U.ntile, an[2 * U.layer], cb, sb, I, P, TC,
-                                       U.layer == 0, amp);
+                tile_run_tb<true>(tb, tile, psi, U.idx, U.ntile, an[2 * U.layer], cb, sb, I, &P, &TC, gen, amp);
             } else {
                 for (uint32_t j = 0; j < U.ntile; ++j) {
                     const uint64_t base = (uint64_t(U.idx) + j) << tb;
                     for (uint32_t i = threadIdx.x; i < C; i += 256)
-                        tile[i] = U.layer ? ldamp<float, true>(psi + ((base + i) ^ arena_xor(base)))
-                                          : make_float2(amp, 0.f);
+                        tile[i] = gen ? make_float2(amp, 0.f)
+                                      : ldamp<float, true>(psi + ((base + i) ^ arena_xor(base)));
                     __syncthreads();
-                    if (cost) lds_cost(tile, tb, base, an[2 * U.layer], I, P);
+                    lds_cost(tile, tb, base, an[2 * U.layer], I, P);
                     lds_rx(tile, tb, float(cb), float(sb));
                     for (uint32_t i = threadIdx.x; i < C; i += 256)
                         stamp<float, true>(psi + ((base + i) ^ arena_xor(base)), tile[i]);
                     __syncthreads();
                 }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Rx-only tile units (kTileRxOnly): Rx(2 beta) on the tile qubits of a run
+// of tiles, the layer's cost diagonal having run in a mixer pass. Only the
+// LDS tile is staged (no instance terms), and at most 128 VGPRs: four
+// blocks per CU instead of the three that the cost kernel's 41 KiB of LDS
+// and 168 registers allow.
+__global__ __launch_bounds__(256, 4) void k_qaoa_tile_rx(float2* __restrict__ arena, const QInst* __restrict__ insts,
+                                                      const QUnit* __restrict__ units, uint32_t nunits,
+                                                      uint32_t* __restrict__ ticket, const double* __restrict__ ang,
+                                                      uint32_t p) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    float2* tile = reinterpret_cast<float2*>(smem_raw);
+    __shared__ uint32_t su;
+    for (;;) {
+        if (threadIdx.x == 0) su = atomicAdd(ticket, 1u);
+        __syncthreads();
+        const uint32_t u = su;
+        __syncthreads();
+        if (u >= nunits) break;  // every wave of the block leaves together
+        const QUnit U = units[u];
+        const QInst I = insts[U.inst];
+        float2* psi = arena + I.off;
+        const uint32_t tb = I.tbits, C = 1u << tb;
+        double sb, cb;
+        sincos(ang[(size_t(U.inst) * p + U.layer) * 2 + 1], &sb, &cb);
+        if (tb >= 8) {
+            tile_run_tb<false>(tb, tile, psi, U.idx, U.ntile, 0.0, cb, sb, I, nullptr, nullptr, false, 0.f);
+        } else {
+            for (uint32_t j = 0; j < U.ntile; ++j) {
+                const uint64_t base = (uint64_t(U.idx) + j) << tb;
+                for (uint32_t i = threadIdx.x; i < C; i += 256)
+                    tile[i] = ldamp<float, true>(psi + ((base + i) ^ arena_xor(base)));
+                __syncthreads();
+                lds_rx(tile, tb, float(cb), float(sb));
+                for (uint32_t i = threadIdx.x; i < C; i += 256)
+                    stamp<float, true>(psi + ((base + i) ^ arena_xor(base)), tile[i]);
+                __syncthreads();
             }
         }
         __syncthreads();
@@ -778,7 +816,8 @@ __global__ __launch_bounds__(256) void k_qaoa_mix(float2* __restrict__ arena, co
         }
         const double* an = ang + size_t(U.inst) * p * 2;
         // MG: gamma and beta of layer l + 1
-        const double gn = MG ? an[2 * (U.layer + 1)] : 0.0;
+        const uint32_t ln = U.layer + 1;
+        const double gn = MG ? an[2 * ln] : 0.0;
         if (MG) {
             for (uint32_t m = threadIdx.x; m < uint32_t(D); m += 256) {
                 double v = 0;  // Q(m) - Q(0): -2 J for each inside edge whose spins differ in m
@@ -795,7 +834,7 @@ __global__ __launch_bounds__(256) void k_qaoa_mix(float2* __restrict__ arena, co
         float2* psi = arena + I.off;
         double sb, cb, sb2 = 0, cb2 = 1;
         sincos(an[2 * U.layer + 1], &sb, &cb);
-        if (MG) sincos(an[2 * (U.layer + 1) + 1], &sb2, &cb2);
+        if (MG) sincos(an[2 * ln + 1], &sb2, &cb2);
         const float fc = float(cb), fs = float(sb), fc2 = float(cb2), fs2 = float(sb2);
         const uint64_t norb = (1ull << I.n) >> g;
         double acc = 0;
@@ -954,7 +993,7 @@ struct qcs_qaoa {
     uint32_t* d_large = nullptr;
     uint32_t nlarge = 0;
     // launch plan: one persistent launch per entry, in order
-    enum : uint8_t { L_SMALL = 0, L_TILE = 1, L_MIX = 2 };
+    enum : uint8_t { L_SMALL = 0, L_TILE = 1, L_MIX = 2, L_TILE_RX = 3 };
     struct Launch {
         uint8_t kind, g, mode;  // mode: MIX_* for mixer launches
         uint32_t nunits;
@@ -998,6 +1037,12 @@ static int launch_sequence(qcs_qaoa* q) {
         const auto& L = q->plan[k];
         const unsigned grid = std::min<uint32_t>(L.nunits, 2048);
         uint32_t* tk = q->d_tickets + k;
+        if (L.kind == qcs_qaoa::L_TILE_RX) {
+            const size_t shm = sizeof(float2) * tile_lds_elems(q->tmax);
+            hipLaunchKernelGGL(k_qaoa_tile_rx, dim3(grid), dim3(256), shm, q->stream, q->arena, q->d_inst, L.units,
+                               L.nunits, tk, q->d_ang, q->p);
+            continue;
+        }
         if (L.kind == qcs_qaoa::L_SMALL || L.kind == qcs_qaoa::L_TILE) {
             uint32_t tb = q->tmax;
             if (L.kind == qcs_qaoa::L_SMALL) {
@@ -1053,27 +1098,31 @@ struct PassSpec {
 // The passes of one evaluation of an instance of n > T qubits whose qubits
 // above the tile form ng mixer groups. ng = 1: per layer the tile pass (cost
 // diagonal, Rx on the tile qubits) and the group's Rx. ng >= 2: the last two
-// groups take turns at ending a layer with a merged pass (Rx of layer l,
-// cost of layer l + 1, Rx of layer l + 1); the next layer's tile pass then
-// runs without the cost and that layer skips the merged group. That is
-// p ng + 1 passes instead of p (ng + 1): 9 instead of 12 for two groups at
-// depth 4. Every layer's cost runs before its rotations, and every rotation
-// of a layer before the next layer's cost.
+// groups X = ng - 1 and Y = ng - 2 take turns (X first) at ending a layer
+// with a merged pass (Rx of layer l, cost of layer l + 1, Rx of layer l + 1);
+// the next layer's tile pass then runs without the cost and that layer
+// skips the merged group. That is p ng + 1 passes instead of p (ng + 1): 9
+// instead of 12 for two groups at depth 4. Every layer's cost runs before
+// its rotations, and every rotation of a layer before the next layer's cost.
+// (Starting instead with a mixer pass that generates |+...+>, applies layer
+// 0's cost and its group's Rx, so that every tile pass only rotates, takes
+// as many passes and measured 1.5 ms slower per 1024-instance batch.)
 static std::vector<PassSpec> instance_passes(uint32_t n, uint32_t Tm, uint32_t G, uint32_t p) {
     const uint32_t ng = (n - Tm + G - 1) / G;
     auto width = [&](uint32_t j) { return uint8_t(j == ng - 1 ? (n - Tm) - G * (ng - 1) : G); };
+    auto mix = [&](uint32_t j, uint8_t mode, uint32_t l) {
+        return PassSpec{qcs_qaoa::L_MIX, width(j), mode, Tm + j * G, l};
+    };
     std::vector<PassSpec> v;
     int merged = -1;  // the group whose Rx of this layer ran in the previous layer's merged pass
     for (uint32_t l = 0; l < p; ++l) {
         v.push_back(PassSpec{qcs_qaoa::L_TILE, 0, uint8_t(merged >= 0 ? kTileRxOnly : 0), 0, l});
-        const int z = ng >= 2 ? int(l % 2 == 0 ? ng - 1 : ng - 2) : int(ng - 1);
+        const uint32_t z = ng >= 2 ? (l % 2 == 0 ? ng - 1 : ng - 2) : 0;
         for (uint32_t j = 0; j < ng; ++j)
-            if (int(j) != merged && int(j) != z)
-                v.push_back(PassSpec{qcs_qaoa::L_MIX, width(j), MIX_PLAIN, Tm + j * G, l});
+            if (int(j) != merged && j != z) v.push_back(mix(j, MIX_PLAIN, l));
         const bool last = l + 1 == p;
-        const uint8_t zm = last ? MIX_ENERGY : (ng >= 2 ? MIX_MERGE : MIX_PLAIN);
-        v.push_back(PassSpec{qcs_qaoa::L_MIX, width(uint32_t(z)), zm, Tm + uint32_t(z) * G, l});
-        merged = (!last && ng >= 2) ? z : -1;
+        v.push_back(mix(z, last ? MIX_ENERGY : (ng >= 2 ? MIX_MERGE : MIX_PLAIN), l));
+        merged = (!last && ng >= 2) ? int(z) : -1;
     }
     return v;
 }
@@ -1124,14 +1173,17 @@ static int build_plan(qcs_qaoa* q) {
             const PassSpec& S = passes[k][s];
             const uint32_t i = large[k], n = q->nq[i];
             const bool tile = S.kind == qcs_qaoa::L_TILE;
+            // Rx-only tile passes run on their own kernel
+            const uint8_t kind = tile && (S.mode & kTileRxOnly) ? uint8_t(qcs_qaoa::L_TILE_RX) : S.kind;
             const uint8_t g = tile ? 0 : S.g, md = tile ? 0 : S.mode;
             size_t j = first;
-            while (j < pend.size() && !(pend[j].kind == S.kind && pend[j].g == g && pend[j].mode == md)) ++j;
-            if (j == pend.size()) pend.push_back(Pending{S.kind, g, md, {}});
+            while (j < pend.size() && !(pend[j].kind == kind && pend[j].g == g && pend[j].mode == md)) ++j;
+            if (j == pend.size()) pend.push_back(Pending{kind, g, md, {}});
             auto& P = pend[j];
             if (tile) {
-                for (uint32_t t = 0, nt = 1u << (n - Tm); t < nt; t += kTileRun)
-                    P.units.push_back(QUnit{i, t, S.layer, 0, 0, std::min(kTileRun, nt - t), S.mode});
+                const uint32_t run = (S.mode & kTileRxOnly) ? kTileRunRx : kTileRun;
+                for (uint32_t t = 0, nt = 1u << (n - Tm); t < nt; t += run)
+                    P.units.push_back(QUnit{i, t, S.layer, 0, 0, std::min(run, nt - t), S.mode});
             } else {
                 const bool en = S.mode == MIX_ENERGY;
                 const uint64_t chunks = (((1ull << n) >> S.g) + 255) / 256;

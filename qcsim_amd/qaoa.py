@@ -84,7 +84,8 @@ def qaoa_plan(n: int, p: int, tile_bits: int = 12, group: int = 6):
     layer) with kind "lds" (the whole evaluation in LDS), "tile" (mode 0:
     cost then Rx of `layer` on qubits 0..tile_bits-1; 1: Rx only) or "mixer"
     (qubits qlo..qlo+width-1; mode 0: Rx; 1: Rx then the energy; 2: Rx of
-    `layer`, cost of layer + 1, Rx of layer + 1)."""
+    `layer`, cost of layer + 1, Rx of layer + 1). The first pass starts from
+    |+...+>."""
     cnt = C.c_uint32()
     check(lib().qcs_qaoa_plan(n, tile_bits, group, p, None, 0, C.byref(cnt)))
     buf = (C.c_uint32 * (5 * cnt.value))()

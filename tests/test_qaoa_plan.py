@@ -34,8 +34,9 @@ def _replay(plan, n, tb, C, gam, bet):
     for k, (kind, w, mode, qlo, l) in enumerate(plan):
         if kind == "lds":
             return _plain(n, C, gam, bet), 1
+        # only the first pass starts from |+...+> (writes without reading)
+        assert (k == 0) == (kind == "tile" and mode == 0 and l == 0)
         if kind == "tile":
-            assert (k == 0) == (l == 0)  # layer 0's tile pass starts the plan from |+...+>
             if mode == 0:
                 psi = psi * np.exp(-1j * gam[l] * C)
             psi = _rx(psi, n, range(tb), bet[l])
