@@ -39,6 +39,32 @@ def test_batch_matches_oracle(tile_bits, group):
     batch.close()
 
 
+@pytest.mark.parametrize("p", [1, 2, 5])
+def test_merged_schedule_depths_match_oracle(p):
+    """Depths other than 3 and 4 under the merged pass plan (qcs_qaoa_plan):
+    tile bits 6 and width-4 groups give 11..16-qubit instances two or three
+    mixer groups, so the plan alternates merged passes between the last two
+    groups and puts the energy on either of them (p = 1: no merged pass)."""
+    from qcsim_amd.qaoa import qaoa_plan
+
+    sizes = [6, 11, 12, 14, 16]
+    insts = make_batch_instances(100 + p, sizes)
+    plans = [qaoa_plan(n, p, 6, 4) for n in sizes]
+    assert any(any(st[0] == "mixer" and st[2] == 2 for st in pl) for pl in plans) == (p > 1)
+    rng = np.random.default_rng(50 + p)
+    G = rng.uniform(0, np.pi, (len(insts), p))
+    B = rng.uniform(0, np.pi, (len(insts), p))
+    batch = QaoaBatch(insts, p, tile_bits=6, group=4)
+    E = batch.evaluate(G, B)
+    for i, inst in enumerate(insts):
+        e_ref, psi_ref = oracle_eval(inst, G[i], B[i])
+        scale = 1 + np.sum(np.abs(inst.J)) + np.sum(np.abs(inst.h))
+        assert abs(E[i] - e_ref) < 2e-5 * scale, (inst.n, E[i], e_ref)
+        err = np.max(np.abs(batch.state(i) - psi_ref))
+        assert err < 3e-6, (inst.n, err)
+    batch.close()
+
+
 def test_graph_replays_match_eager():
     insts = make_batch_instances(9, [6, 12, 14, 16, 18, 20])
     p = 4
@@ -77,7 +103,7 @@ def test_maxcut_known_answer_with_plain_optimiser():
 @pytest.mark.parametrize("tile_bits,group", [(12, 6), (11, 5)])
 def test_config_sizes_match_oracle(tile_bits, group):
     """The batched-QAOA workload's largest instances at its depth: n = 20, 22, 24
-    (12 register passes per evaluation with width-6 mixer passes at tile 12)
+    (9 passes per evaluation with width-6 mixer passes at tile 12)
     at p = 4, states and energies against the oracle's gate-by-gate Exec
     (IsingModel.h:485-567) and exact energy (:106-134)."""
     insts = make_batch_instances(77 + tile_bits, [20, 22, 24])
@@ -98,8 +124,8 @@ def test_config_sizes_match_oracle(tile_bits, group):
 
 
 def test_26_qubit_instance_matches_oracle():
-    """An instance above 24 qubits: three mixer passes per layer at tile 12 /
-    group 6 (qubits 12..17, 18..23, 24..25) and a stored order that folds
+    """An instance above 24 qubits: three mixer groups at tile 12 / group 6
+    (qubits 12..17, 18..23, 24..25) and a stored order that folds
     both of its high 6-bit fields (bits 18.. and 24..); p = 2, with a 16-qubit
     instance in the same batch."""
     insts = make_batch_instances(2026, [16, 26])
