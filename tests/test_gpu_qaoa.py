@@ -42,12 +42,13 @@ def test_batch_matches_oracle(tile_bits, group):
 @pytest.mark.parametrize("p", [1, 2, 5])
 def test_merged_schedule_depths_match_oracle(p):
     """Depths other than 3 and 4 under the merged pass plan (qcs_qaoa_plan):
-    tile bits 6 and width-4 groups give 11..16-qubit instances two or three
-    mixer groups, so the plan alternates merged passes between the last two
-    groups and puts the energy on either of them (p = 1: no merged pass)."""
+    tile bits 6 and width-4 groups give 12..16-qubit instances two or three
+    mixer groups (10 qubits: one), so the plan alternates merged passes
+    between the last two groups and puts the energy on either of them (p = 1:
+    no merged pass)."""
     from qcsim_amd.qaoa import qaoa_plan
 
-    sizes = [6, 11, 12, 14, 16]
+    sizes = [6, 10, 12, 14, 16]
     insts = make_batch_instances(100 + p, sizes)
     plans = [qaoa_plan(n, p, 6, 4) for n in sizes]
     assert any(any(st[0] == "mixer" and st[2] == 2 for st in pl) for pl in plans) == (p > 1)
