@@ -238,11 +238,23 @@ __device__ __forceinline__ void apply_d_half(cv<T>* x, uint64_t base, uint32_t t
     }
 }
 
+// The value of lane l ^ 1: a DPP row move (quad_perm 1, 0, 3, 2) per 32-bit
+// half, in the VALU instead of an LDS-crossbar permute.
+__device__ __forceinline__ float lane_xor1(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double lane_xor1(double v) {
+    const unsigned long long b = __double_as_longlong(v);
+    const unsigned lo = unsigned(__builtin_amdgcn_update_dpp(0, int(unsigned(b)), 0xB1, 0xF, 0xF, false));
+    const unsigned hi = unsigned(__builtin_amdgcn_update_dpp(0, int(unsigned(b >> 32)), 0xB1, 0xF, 0xF, false));
+    return __longlong_as_double((long long)((unsigned long long)hi << 32 | lo));
+}
+
 template <typename T>
 __device__ __forceinline__ void h_pair(cv<T>* x, uint32_t half) {
 #pragma unroll
     for (int c = 0; c < 16; ++c) {
-        const T yx = __shfl_xor(x[c].x, 1, 64), yy = __shfl_xor(x[c].y, 1, 64);
+        const T yx = lane_xor1(x[c].x), yy = lane_xor1(x[c].y);
         if (half) {
             x[c].x = yx - x[c].x;
             x[c].y = yy - x[c].y;
