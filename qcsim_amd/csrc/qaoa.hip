@@ -838,6 +838,11 @@ __global__ __launch_bounds__(256) void k_qaoa_mix(float2* __restrict__ arena, co
         const float fc = float(cb), fs = float(sb), fc2 = float(cb2), fs2 = float(sb2);
         const uint64_t norb = (1ull << I.n) >> g;
         double acc = 0;
+        // MG, E: hz and jzz of the previous orbit's base, updated spin by
+        // spin to the next base (the chunks of a unit differ in one or two
+        // spins) instead of a full evaluation per orbit
+        double hz = 0, jzz = 0;
+        uint64_t prev = 0;
 #pragma unroll 1
         for (uint32_t r = 0; r < kMixChunks; ++r) {
             const uint64_t o = (uint64_t(U.idx) + r) * 256 + threadIdx.x;
@@ -866,8 +871,17 @@ __global__ __launch_bounds__(256) void k_qaoa_mix(float2* __restrict__ arena, co
             //       sum_{k: c_k = 1} fk[k] + qe[c].
             double tb0 = 0, fk[g];
             if (MG || E) {
-                double hz, jzz;
-                ising_full(base, I.n, I.ne, P, hz, jzz);
+                if (r == 0) {
+                    ising_full(base, I.n, I.ne, P, hz, jzz);
+                } else {
+                    uint64_t s = prev;
+                    for (uint64_t d = base ^ prev; d; d &= d - 1) {
+                        const uint32_t q = uint32_t(__builtin_ctzll(d));
+                        ising_flip(s, q, P, hz, jzz);
+                        s ^= 1ull << q;
+                    }
+                }
+                prev = base;
                 tb0 = MG ? cost_of(hz, jzz) : energy_of(hz, jzz) - qe[0];
 #pragma unroll
                 for (int k = 0; k < g; ++k) {
