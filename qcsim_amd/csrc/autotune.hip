@@ -14,9 +14,11 @@
 // the 30 targets at 30 qubits; otherwise (complex64, where 8-byte lanes
 // stream slower, and the other kernels) plain loads for qubits 0-2,
 // nontemporal above, both with two orbits per thread and XCD-ordered blocks.
+#include <algorithm>
 #include <atomic>
 #include <functional>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 #include "common.h"
@@ -242,15 +244,24 @@ int pick_fusion(qcs_register* r, const qcs_gate* gates, uint32_t n, std::vector<
     for (uint32_t i = 0; i < n; ++i) best += block_cost(gates[i], t);
     out.assign(gates, gates + n);
     *width = 0;
-    std::vector<qcs_gate> f;
-    for (uint32_t w = 2; w <= 5 && w <= r->n; ++w) {
-        if (const int e = fuse(gates, n, w, f)) return e;
+    // the widths are fused on host threads of their own (the pass is host
+    // code and shares no state between calls), then priced in order
+    const uint32_t wmax = std::min<uint32_t>(5, r->n);
+    std::vector<std::vector<qcs_gate>> f(6);
+    int err[6] = {0, 0, 0, 0, 0, 0};
+    {
+        std::vector<std::thread> pool;
+        for (uint32_t w = 2; w <= wmax; ++w) pool.emplace_back([&, w] { err[w] = fuse(gates, n, w, f[w]); });
+        for (auto& th : pool) th.join();
+    }
+    for (uint32_t w = 2; w <= wmax; ++w) {
+        if (err[w]) return fuse(gates, n, w, f[w]);  // again on this thread, for its error message
         double c = 0;
-        for (const qcs_gate& g : f) c += block_cost(g, t);
+        for (const qcs_gate& g : f[w]) c += block_cost(g, t);
         if (c < best) {
             best = c;
             *width = w;
-            out.swap(f);
+            out.swap(f[w]);
         }
     }
     return QCS_OK;
