@@ -3,7 +3,7 @@
 // The reference builds QFT/IQFT from Hadamards, controlled phase shifts and a
 // SWAP network (QuantumFourierTransform.h:35-87, QubitsSwapper.h:23-33): at
 // n = 30 that is 480 full passes over the state. Here the same operator is
-// applied in about n/5 + 2 passes:
+// applied in about (n - 12)/6 + 2 passes (5 at n = 30):
 //
 //   The controlled phase shifts between target `cur` and every control below
 //   it (phase pi/2, pi/4, ... for distance 1, 2, ...) multiply to one
@@ -12,10 +12,11 @@
 //   H(eq), D_eq, H(eq-1), D_(eq-1), ..., D_(sq+1), H(sq) and the inverse is
 //   H(sq), D*_(sq+1), H(sq+1), ..., D*_eq, H(eq) (after the swaps).
 //
-//   k_qft_group applies that sequence for up to 5 consecutive targets at once
-//   with the 2^5 amplitudes of each orbit in registers; k_qft_lds applies it
-//   for all targets below 11 inside 2048-amplitude chunks staged in LDS, in
-//   rounds of up to 3 targets held in registers; the
+//   k_qft_group applies that sequence for up to 4 consecutive targets at once
+//   with the 2^4 amplitudes of each orbit in registers, k_qft_gpair for 5 or
+//   6 with each orbit split over a lane pair; k_qft_lds applies it for all
+//   targets below 12 inside 4096-amplitude chunks staged in LDS, in rounds
+//   of up to 3 targets held in registers; the
 //   SWAP network is one bit-reversal pass (k_bitrev_tile, LDS-tiled, or
 //   k_bitrev_pairs when runs of >= 8 amplitudes stay contiguous).
 //
@@ -36,9 +37,9 @@ namespace qcs {
 constexpr int kTwBits = 12;
 constexpr int kTwLen = 1 << kTwBits;
 constexpr int kPhaseBits = 36;  // phases are multiples of pi / 2^36
-constexpr int kLdsLog = 11;     // k_qft_lds chunk: 2048 amplitudes
+constexpr int kLdsLog = 12;     // k_qft_lds chunk: 4096 amplitudes
 constexpr int kLdsRound = 3;    // targets per k_qft_lds round (8 amplitudes per thread)
-constexpr int kGroupMax = 5;
+constexpr int kGroupMax = 6;   // widest register group; roots exp(i pi j / 2^kGroupMax)
 
 // ---- twiddle tables (process-wide, one device copy per device) -------------
 static std::mutex g_tw_mu;
@@ -211,30 +212,29 @@ __global__ __launch_bounds__(256) void k_qft_group(cv<T>* __restrict__ psi, uint
     for (int c = 0; c < D; ++c) stamp<T, true>(p0 + c * stride, scaled<T>(x[c], hscale<T>(G)));
 }
 
-// Five targets per pass with each 32-amplitude orbit split over a lane pair:
-// lane half h holds orbit elements c + 16 h (c < 16), so a thread keeps 16
-// amplitudes in registers; the target whose bit is the pair bit (local bit
-// 4) exchanges values with the partner lane.
-template <typename T>
+// G = 5 or 6 targets per pass with each 2^G-amplitude orbit split over a
+// lane pair: lane half h holds orbit elements c + 2^(G-1) h (c < 2^(G-1)), so
+// a thread keeps 16 or 32 amplitudes in registers; the target whose bit is
+// the pair bit (local bit G - 1) exchanges values with the partner lane.
+template <typename T, int G>
 __device__ __forceinline__ void apply_d_half(cv<T>* x, uint64_t base, uint32_t t, uint32_t half, const QftArgs& a,
                                              const double2* __restrict__ tw, const QftRoots& R, bool conj) {
+    constexpr int H = 1 << (G - 1);
     const uint32_t k = t + 1 - a.sq;
     const uint32_t nlow = a.lo - a.sq;
     const uint64_t Lf = nlow ? ((base >> a.sq) & ((1ull << nlow) - 1)) : 0;
     const double2 F = phase_pi(tw, Lf, k);
     const uint32_t tl = t - a.lo;
-    const bool top_fixed = (t + 1 >= a.lo + kGroupMax);
+    const bool top_fixed = (t + 1 >= a.lo + G);
     if (top_fixed && !((base >> (t + 1)) & 1)) return;
 #pragma unroll
-    for (int c = 0; c < 16; ++c) {
-        const uint32_t cl = uint32_t(c) | (half << 4);
+    for (int c = 0; c < H; ++c) {
+        const uint32_t cl = uint32_t(c) | (half << (G - 1));
         if (!top_fixed && !((cl >> (tl + 1)) & 1)) continue;
         const uint32_t Gt = cl & ((2u << tl) - 1);
-        const double2 p = R.w[Gt << (kGroupMax - 1 - tl)];
-        double2 f;
-        f.x = p.x * F.x - p.y * F.y;
-        f.y = p.x * F.y + p.y * F.x;
-        x[c] = mul_ph<T>(x[c], f, conj);
+        // x F, then times the root (a wave-uniform constant): no per-element
+        // phase is formed, which keeps 32 amplitudes per lane within budget
+        x[c] = mul_ph<T>(mul_ph<T>(x[c], F, conj), R.w[Gt << (kGroupMax - 1 - tl)], conj);
     }
 }
 
@@ -250,10 +250,10 @@ __device__ __forceinline__ double lane_xor1(double v) {
     return __longlong_as_double((long long)((unsigned long long)hi << 32 | lo));
 }
 
-template <typename T>
+template <typename T, int H>
 __device__ __forceinline__ void h_pair(cv<T>* x, uint32_t half) {
 #pragma unroll
-    for (int c = 0; c < 16; ++c) {
+    for (int c = 0; c < H; ++c) {
         const T yx = lane_xor1(x[c].x), yy = lane_xor1(x[c].y);
         if (half) {
             x[c].x = yx - x[c].x;
@@ -265,56 +265,61 @@ __device__ __forceinline__ void h_pair(cv<T>* x, uint32_t half) {
     }
 }
 
-template <typename T, bool INV>
-__global__ __launch_bounds__(256) void k_qft_group5(cv<T>* __restrict__ psi, uint64_t norb, QftArgs a,
-                                                    const double2* __restrict__ tw, const QftRoots R) {
+template <typename T, int G, bool INV>
+__global__ __launch_bounds__(256) void k_qft_gpair(cv<T>* __restrict__ psi, uint64_t norb, QftArgs a,
+                                                   const double2* __restrict__ tw, const QftRoots R) {
+    constexpr int H = 1 << (G - 1);
     const uint64_t gt = uint64_t(blockIdx.x) * 256 + threadIdx.x;
     const uint64_t o = gt >> 1;
     const uint32_t half = threadIdx.x & 1;
     if (o >= norb) return;  // both lanes of a pair leave together
-    const uint64_t base = insert_run(o, a.lo, kGroupMax);
-    cv<T>* const p0 = psi + base + (uint64_t(half) << (a.lo + 4));
+    const uint64_t base = insert_run(o, a.lo, G);
+    cv<T>* const p0 = psi + base + (uint64_t(half) << (a.lo + G - 1));
     const uint64_t stride = 1ull << a.lo;
-    cv<T> x[16];
+    cv<T> x[H];
 #pragma unroll
-    for (int c = 0; c < 16; ++c) x[c] = ldamp<T, kStagedNtLoads>(p0 + c * stride);
+    for (int c = 0; c < H; ++c) x[c] = ldamp<T, kStagedNtLoads>(p0 + c * stride);
     if (!INV) {
 #pragma unroll
-        for (int tl = kGroupMax - 1; tl >= 0; --tl) {
+        for (int tl = G - 1; tl >= 0; --tl) {
             const uint32_t t = a.lo + tl;
-            if (t < a.eq) apply_d_half<T>(x, base, t, half, a, tw, R, false);
-            if (tl == kGroupMax - 1)
-                h_pair<T>(x, half);
+            if (t < a.eq) apply_d_half<T, G>(x, base, t, half, a, tw, R, false);
+            if (tl == G - 1)
+                h_pair<T, H>(x, half);
             else
-                apply_h<T, 4>(x, tl);
+                apply_h<T, G - 1>(x, tl);
+            // keep each target's phase loads next to their use (register
+            // pressure of the 2^(G-1) amplitudes per lane)
+            if constexpr (G == 6) __builtin_amdgcn_sched_barrier(0);
         }
     } else {
 #pragma unroll
-        for (int tl = 0; tl < kGroupMax; ++tl) {
+        for (int tl = 0; tl < G; ++tl) {
             const uint32_t t = a.lo + tl;
-            if (tl == kGroupMax - 1)
-                h_pair<T>(x, half);
+            if (tl == G - 1)
+                h_pair<T, H>(x, half);
             else
-                apply_h<T, 4>(x, tl);
-            if (t < a.eq) apply_d_half<T>(x, base, t, half, a, tw, R, true);
+                apply_h<T, G - 1>(x, tl);
+            if (t < a.eq) apply_d_half<T, G>(x, base, t, half, a, tw, R, true);
+            if constexpr (G == 6) __builtin_amdgcn_sched_barrier(0);
         }
     }
 #pragma unroll
-    for (int c = 0; c < 16; ++c) stamp<T, true>(p0 + c * stride, scaled<T>(x[c], hscale<T>(kGroupMax)));
+    for (int c = 0; c < H; ++c) stamp<T, true>(p0 + c * stride, scaled<T>(x[c], hscale<T>(G)));
 }
 
 // LDS position of chunk element i: bits 0..3 XOR bits 4..7, so that the
 // 16 lanes of one access phase hit different banks for every round layout.
 __device__ __forceinline__ uint32_t swz(uint32_t i) { return i ^ ((i >> 4) & 15u); }
 
-// x * exp(i pi J / 32) (conjugate with CONJ) for a root index J known at
-// compile time: J = 0 is the identity, J = 16 is a multiplication by i.
+// x * exp(i pi J / 64) (conjugate with CONJ) for a root index J known at
+// compile time: J = 0 is the identity, J = 32 is a multiplication by i.
 template <typename T, int J, bool CONJ>
 __device__ __forceinline__ cv<T> mul_root(cv<T> x, const QftRoots& R) {
     static_assert(J >= 0 && J < (1 << kGroupMax), "root index");
     if constexpr (J == 0) {
         return x;
-    } else if constexpr (J == 16) {
+    } else if constexpr (J == (1 << (kGroupMax - 1))) {
         cv<T> r;
         r.x = CONJ ? x.y : -x.y;
         r.y = CONJ ? -x.x : x.x;
@@ -343,6 +348,13 @@ __device__ __forceinline__ void apply_p(cv<T>* x, const QftRoots& R) {
             QCS_ROOT_CASE(17) QCS_ROOT_CASE(18) QCS_ROOT_CASE(19) QCS_ROOT_CASE(20) QCS_ROOT_CASE(21)
             QCS_ROOT_CASE(22) QCS_ROOT_CASE(23) QCS_ROOT_CASE(24) QCS_ROOT_CASE(25) QCS_ROOT_CASE(26)
             QCS_ROOT_CASE(27) QCS_ROOT_CASE(28) QCS_ROOT_CASE(29) QCS_ROOT_CASE(30) QCS_ROOT_CASE(31)
+            QCS_ROOT_CASE(32) QCS_ROOT_CASE(33) QCS_ROOT_CASE(34) QCS_ROOT_CASE(35) QCS_ROOT_CASE(36)
+            QCS_ROOT_CASE(37) QCS_ROOT_CASE(38) QCS_ROOT_CASE(39) QCS_ROOT_CASE(40) QCS_ROOT_CASE(41)
+            QCS_ROOT_CASE(42) QCS_ROOT_CASE(43) QCS_ROOT_CASE(44) QCS_ROOT_CASE(45) QCS_ROOT_CASE(46)
+            QCS_ROOT_CASE(47) QCS_ROOT_CASE(48) QCS_ROOT_CASE(49) QCS_ROOT_CASE(50) QCS_ROOT_CASE(51)
+            QCS_ROOT_CASE(52) QCS_ROOT_CASE(53) QCS_ROOT_CASE(54) QCS_ROOT_CASE(55) QCS_ROOT_CASE(56)
+            QCS_ROOT_CASE(57) QCS_ROOT_CASE(58) QCS_ROOT_CASE(59) QCS_ROOT_CASE(60) QCS_ROOT_CASE(61)
+            QCS_ROOT_CASE(62) QCS_ROOT_CASE(63)
 #undef QCS_ROOT_CASE
             default: break;
         }
@@ -365,6 +377,13 @@ __device__ __forceinline__ void apply_p_top(cv<T>* x, const QftRoots& R) {
             QCS_ROOT_CASE(17) QCS_ROOT_CASE(18) QCS_ROOT_CASE(19) QCS_ROOT_CASE(20) QCS_ROOT_CASE(21)
             QCS_ROOT_CASE(22) QCS_ROOT_CASE(23) QCS_ROOT_CASE(24) QCS_ROOT_CASE(25) QCS_ROOT_CASE(26)
             QCS_ROOT_CASE(27) QCS_ROOT_CASE(28) QCS_ROOT_CASE(29) QCS_ROOT_CASE(30) QCS_ROOT_CASE(31)
+            QCS_ROOT_CASE(32) QCS_ROOT_CASE(33) QCS_ROOT_CASE(34) QCS_ROOT_CASE(35) QCS_ROOT_CASE(36)
+            QCS_ROOT_CASE(37) QCS_ROOT_CASE(38) QCS_ROOT_CASE(39) QCS_ROOT_CASE(40) QCS_ROOT_CASE(41)
+            QCS_ROOT_CASE(42) QCS_ROOT_CASE(43) QCS_ROOT_CASE(44) QCS_ROOT_CASE(45) QCS_ROOT_CASE(46)
+            QCS_ROOT_CASE(47) QCS_ROOT_CASE(48) QCS_ROOT_CASE(49) QCS_ROOT_CASE(50) QCS_ROOT_CASE(51)
+            QCS_ROOT_CASE(52) QCS_ROOT_CASE(53) QCS_ROOT_CASE(54) QCS_ROOT_CASE(55) QCS_ROOT_CASE(56)
+            QCS_ROOT_CASE(57) QCS_ROOT_CASE(58) QCS_ROOT_CASE(59) QCS_ROOT_CASE(60) QCS_ROOT_CASE(61)
+            QCS_ROOT_CASE(62) QCS_ROOT_CASE(63)
 #undef QCS_ROOT_CASE
             default: break;
         }
@@ -661,13 +680,18 @@ static int launch_group(qcs_register* r, const double2* tw, uint32_t sq, uint32_
         case 3: QCS_G(3); break;
         case 4: QCS_G(4); break;
         default: {
-            const unsigned grid5 = unsigned((2 * norb + 255) / 256);
-            if (inv)
-                hipLaunchKernelGGL((k_qft_group5<T, true>), dim3(grid5), dim3(256), 0, r->stream, psi, norb, a, tw,
-                                   roots);
-            else
-                hipLaunchKernelGGL((k_qft_group5<T, false>), dim3(grid5), dim3(256), 0, r->stream, psi, norb, a, tw,
-                                   roots);
+            // 5 or 6 targets: orbits split over lane pairs
+            const unsigned gridp = unsigned((2 * norb + 255) / 256);
+#define QCS_GP(GG, II)                                                                                           \
+    hipLaunchKernelGGL((k_qft_gpair<T, GG, II>), dim3(gridp), dim3(256), 0, r->stream, psi, norb, a, tw, roots)
+            if (g == 5) {
+                if (inv) QCS_GP(5, true);
+                else QCS_GP(5, false);
+            } else {
+                if (inv) QCS_GP(6, true);
+                else QCS_GP(6, false);
+            }
+#undef QCS_GP
             break;
         }
     }

@@ -317,6 +317,26 @@ def test_qft_fused_paths(sq, eq, precision, tol):
             assert err < tol, (sq, eq, precision, inverse, swap, err)
 
 
+@pytest.mark.parametrize("precision,tol", [("c128", TOL64), ("c64", TOL32)])
+@pytest.mark.parametrize("sq,eq", [(0, 19), (0, 16), (1, 18), (5, 19), (12, 19)])
+def test_qft_register_groups_20_qubits(sq, eq, precision, tol):
+    """The register-group passes of widths 5 and 6 (orbits split over lane
+    pairs) beside the 12-target LDS pass, on a random 20-qubit state against
+    the oracle's gate sequence: (0, 19) takes groups of 6 and 2, (0, 16) one
+    of 5, (1, 18) one of 6 plus 1, (5, 19) the LDS pass on 5..11 and a group
+    of 6 plus 2, (12, 19) groups of 6 and 2 with no LDS pass."""
+    n = 20
+    rng = np.random.default_rng(sq * 37 + eq)
+    psi = random_state(rng, n)
+    for inverse in (False, True):
+        for swap in (True, False):
+            g, o = pair(n, psi, precision)
+            g.qft(sq, eq, swap, inverse)
+            o.qft(sq, eq, swap, inverse)
+            err = np.max(np.abs(g.getRegisterStorage() - o.state()))
+            assert err < tol, (sq, eq, precision, inverse, swap, err)
+
+
 def _product_state(rng, n):
     """Haar U(2) on every qubit of |0...0>: psi[s] = prod_q u_q[bit_q(s)]."""
     us = [G.haar_u2(rng) for _ in range(n)]
