@@ -38,6 +38,7 @@ constexpr int kTwBits = 12;
 constexpr int kTwLen = 1 << kTwBits;
 constexpr int kPhaseBits = 36;  // phases are multiples of pi / 2^36
 constexpr int kLdsLog = 12;     // k_qft_lds chunk: 4096 amplitudes
+constexpr int kLdsThreads = 512; // k_qft_lds block: one 8-amplitude orbit per thread and round
 constexpr int kLdsRound = 3;    // targets per k_qft_lds round (8 amplitudes per thread)
 constexpr int kGroupMax = 6;   // widest register group; roots exp(i pi j / 2^kGroupMax)
 
@@ -420,7 +421,7 @@ __device__ void lds_round(cv<T>* s, uint64_t cbase, uint32_t clog, const QftArgs
     const bool has_top = top < a.eq;  // the top target has a phase (all lower ones do)
     const uint32_t nlow = a.lo - a.sq;
     const uint32_t ktop = top + 1 - a.sq;
-    for (uint32_t o = threadIdx.x; o < norb; o += 256) {
+    for (uint32_t o = threadIdx.x; o < norb; o += kLdsThreads) {
         const uint32_t lb = uint32_t(insert_run(o, a.lo, G));
         cv<T> x[D];
 #pragma unroll
@@ -494,23 +495,23 @@ __device__ void lds_round_g(cv<T>* s, uint64_t cbase, uint32_t clog, const QftAr
 // trip per round instead of one per target. One chunk per block (one-shot
 // grid, in launches of at most kMaxBlocks blocks from chunk boff).
 template <typename T, bool INV>
-__global__ __launch_bounds__(256) void k_qft_lds(cv<T>* __restrict__ psi, uint32_t clog, uint64_t boff, QftArgs a,
+__global__ __launch_bounds__(kLdsThreads) void k_qft_lds(cv<T>* __restrict__ psi, uint32_t clog, uint64_t boff, QftArgs a,
                                                  uint32_t hi, const double2* __restrict__ tw, const QftRoots R) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     cv<T>* s = reinterpret_cast<cv<T>*>(smem_raw);
-    constexpr int EPT = (1 << kLdsLog) / 256;  // chunk elements per thread
+    constexpr int EPT = (1 << kLdsLog) / kLdsThreads;  // chunk elements per thread
     const uint32_t C = 1u << clog;
     const uint64_t cbase = (boff + blockIdx.x) << clog;
     {
         cv<T> v[EPT];
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
-            const uint32_t i = threadIdx.x + 256u * k;
+            const uint32_t i = threadIdx.x + uint32_t(kLdsThreads) * k;
             if (i < C) v[k] = ldamp<T, kStagedNtLoads>(psi + cbase + i);
         }
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
-            const uint32_t i = threadIdx.x + 256u * k;
+            const uint32_t i = threadIdx.x + uint32_t(kLdsThreads) * k;
             if (i < C) s[swz(i)] = v[k];
         }
         __syncthreads();
@@ -533,7 +534,7 @@ __global__ __launch_bounds__(256) void k_qft_lds(cv<T>* __restrict__ psi, uint32
         }
     }
     const T hs = hscale<T>(hi - a.sq + 1);
-    for (uint32_t i = threadIdx.x; i < C; i += 256) stamp<T, true>(psi + cbase + i, scaled<T>(s[swz(i)], hs));
+    for (uint32_t i = threadIdx.x; i < C; i += kLdsThreads) stamp<T, true>(psi + cbase + i, scaled<T>(s[swz(i)], hs));
 }
 
 // ---- bit reversal of the range [sq, eq] ---------------------------------------
@@ -710,10 +711,10 @@ static int launch_lds(qcs_register* r, const double2* tw, uint32_t sq, uint32_t 
     for (uint64_t boff = 0; boff < nchunks; boff += kMaxBlocks) {
         const unsigned g = unsigned(std::min<uint64_t>(kMaxBlocks, nchunks - boff));
         if (inv)
-            hipLaunchKernelGGL((k_qft_lds<T, true>), dim3(g), dim3(256), shm, r->stream, psi, clog, boff, a, hi, tw,
+            hipLaunchKernelGGL((k_qft_lds<T, true>), dim3(g), dim3(kLdsThreads), shm, r->stream, psi, clog, boff, a, hi, tw,
                                roots);
         else
-            hipLaunchKernelGGL((k_qft_lds<T, false>), dim3(g), dim3(256), shm, r->stream, psi, clog, boff, a, hi, tw,
+            hipLaunchKernelGGL((k_qft_lds<T, false>), dim3(g), dim3(kLdsThreads), shm, r->stream, psi, clog, boff, a, hi, tw,
                                roots);
     }
     return check_launch(r, "k_qft_lds");
