@@ -251,10 +251,12 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 4 : 1) void k_dense_mfma(cv<T
 
 template <typename T, int K>
 static int kdense_mfma(qcs_register* r, const uint32_t* qubits, const double* mat_host) {
-    // 64-orbit tiles (NC = 1). 128-orbit tiles (NC = 2) measured no faster
-    // for 4-qubit blocks at 30 qubits (6.77 against 6.6-6.9 ms per launch)
-    // at lower occupancy, and do not fit the register budget of 5-qubit ones.
-    constexpr int NC = 1, OB = 6;
+    // complex128: 64-orbit tiles (NC = 1); 128-orbit tiles measured no
+    // faster for 4-qubit blocks at 30 qubits (6.77 against 6.6-6.9 ms per
+    // launch) at lower occupancy. complex64: 128-orbit tiles (NC = 2), so
+    // that a tile's runs are at least 128 amplitudes (1 KiB) long: the
+    // 80-layer 32-qubit circuit fused to width 5 took 4.41 s against 4.56 s.
+    constexpr int NC = sizeof(T) == 4 ? 2 : 1, OB = NC == 2 ? 7 : 6;
     constexpr int D = 1 << K;
     // factors Re M, Im M and Re M + Im M, stacked (3 D x D, row-major)
     std::vector<T> R(size_t(3) * D * D);
@@ -314,7 +316,7 @@ static int kdense_mfma(qcs_register* r, const uint32_t* qubits, const double* ma
 
 template <typename T, int K>
 static int kdense_big(qcs_register* r, const uint32_t* qubits, const double* mat_host) {
-    constexpr uint32_t kMinQubits = K + 6;  // one MFMA tile
+    constexpr uint32_t kMinQubits = K + (sizeof(T) == 4 ? 7 : 6);  // one MFMA tile
     if (r->n >= kMinQubits && !r->dense_no_mfma) return kdense_mfma<T, K>(r, qubits, mat_host);
     constexpr int D = 1 << K;
     // matrix converted to T, split re[] then im[], through the staging ring

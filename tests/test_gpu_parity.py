@@ -379,10 +379,13 @@ def test_qft_full_size_properties(n):
 
 
 @pytest.mark.parametrize("precision,tol", [("c128", TOL64), ("c64", 5e-6)])
-@pytest.mark.parametrize("n,k", [(11, 4), (11, 5), (16, 5), (16, 4), (12, 3)])
+@pytest.mark.parametrize("n,k", [(11, 4), (11, 5), (12, 5), (16, 5), (16, 4), (12, 3)])
 def test_dense_blocks_mfma(precision, tol, n, k):
-    """Dense k-qubit blocks through the MFMA kernel (k = 4, 5 when n >= k + 6)
-    at low, high, mixed and reversed qubit orders."""
+    """Dense k-qubit blocks through the MFMA kernel (k = 4, 5 when n >= k + 6
+    for complex128 with 64-orbit tiles, n >= k + 7 for complex64 with 128-orbit
+    tiles; below that the register kernel) at low, high, mixed and reversed
+    qubit orders: (11, 5) is the smallest complex128 MFMA case and a complex64
+    fallback, (12, 5) the smallest complex64 MFMA case."""
     from refmath import random_unitary
 
     rng = np.random.default_rng(100 * n + k)
