@@ -412,21 +412,16 @@ __device__ __forceinline__ double2 cmul2(double2 a, double2 b) {
 // start of the round. Per orbit a table Q[j] = F^(2j + b) (b = the element-
 // independent bit above the group) is built with a few complex products;
 // each element then takes one complex product instead of one per target.
+// The round on one orbit in registers: x[c] is the element base + c 2^lo.
 template <typename T, int G, bool INV>
-__device__ void lds_round(cv<T>* s, uint64_t cbase, uint32_t clog, const QftArgs& a, const double2* __restrict__ tw,
-                          const QftRoots& R) {
+__device__ __forceinline__ void round_orbit(cv<T>* x, uint64_t base, const QftArgs& a,
+                                            const double2* __restrict__ tw, const QftRoots& R) {
     constexpr int D = 1 << G;
-    const uint32_t norb = 1u << (clog - G);
     const uint32_t top = a.lo + G - 1;
     const bool has_top = top < a.eq;  // the top target has a phase (all lower ones do)
     const uint32_t nlow = a.lo - a.sq;
     const uint32_t ktop = top + 1 - a.sq;
-    for (uint32_t o = threadIdx.x; o < norb; o += kLdsThreads) {
-        const uint32_t lb = uint32_t(insert_run(o, a.lo, G));
-        cv<T> x[D];
-#pragma unroll
-        for (int c = 0; c < D; ++c) x[c] = s[swz(lb + (uint32_t(c) << a.lo))];
-        const uint64_t base = cbase | lb;
+    {
         const bool bt = has_top && ((base >> (top + 1)) & 1);
         const uint64_t Lf = nlow ? ((base >> a.sq) & ((1ull << nlow) - 1)) : 0;
         // Q[j] = F^(2j + bt), j < D / 2
@@ -473,6 +468,20 @@ __device__ void lds_round(cv<T>* s, uint64_t cbase, uint32_t clog, const QftArgs
             apply_h<T, G>(x, G - 1);
             if (bt) apply_p_top<T, G, true>(x, R);
         }
+    }
+}
+
+template <typename T, int G, bool INV>
+__device__ void lds_round(cv<T>* s, uint64_t cbase, uint32_t clog, const QftArgs& a, const double2* __restrict__ tw,
+                          const QftRoots& R) {
+    constexpr int D = 1 << G;
+    const uint32_t norb = 1u << (clog - G);
+    for (uint32_t o = threadIdx.x; o < norb; o += kLdsThreads) {
+        const uint32_t lb = uint32_t(insert_run(o, a.lo, G));
+        cv<T> x[D];
+#pragma unroll
+        for (int c = 0; c < D; ++c) x[c] = s[swz(lb + (uint32_t(c) << a.lo))];
+        round_orbit<T, G, INV>(x, cbase | lb, a, tw, R);
 #pragma unroll
         for (int c = 0; c < D; ++c) s[swz(lb + (uint32_t(c) << a.lo))] = x[c];
     }
@@ -502,12 +511,24 @@ __global__ __launch_bounds__(kLdsThreads) void k_qft_lds(cv<T>* __restrict__ psi
     constexpr int EPT = (1 << kLdsLog) / kLdsThreads;  // chunk elements per thread
     const uint32_t C = 1u << clog;
     const uint64_t cbase = (boff + blockIdx.x) << clog;
+    // Forward, full chunk, the top round on the chunk's top 3 bits: a
+    // thread's 8 loaded elements tid + 512 k are one orbit of those bits, so
+    // that round runs in registers before the tile goes to LDS (one LDS round
+    // trip fewer).
+    static_assert(EPT == 8 && kLdsRound == 3, "the loaded elements are one 3-target orbit");
+    const bool reg_top = !INV && clog == uint32_t(kLdsLog) && hi == clog - 1 && hi >= a.sq + 2;
     {
         cv<T> v[EPT];
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
             const uint32_t i = threadIdx.x + uint32_t(kLdsThreads) * k;
             if (i < C) v[k] = ldamp<T, kStagedNtLoads>(psi + cbase + i);
+        }
+        if (reg_top) {
+            QftArgs r = a;
+            r.lo = hi - 2;
+            r.g = 3;
+            round_orbit<T, 3, false>(v, cbase | threadIdx.x, r, tw, R);
         }
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
@@ -517,7 +538,7 @@ __global__ __launch_bounds__(kLdsThreads) void k_qft_lds(cv<T>* __restrict__ psi
         __syncthreads();
     }
     if (!INV) {
-        for (int top = int(hi); top >= int(a.sq); top -= kLdsRound) {
+        for (int top = int(hi) - (reg_top ? kLdsRound : 0); top >= int(a.sq); top -= kLdsRound) {
             QftArgs r = a;
             r.lo = uint32_t(top - (kLdsRound - 1) > int(a.sq) ? top - (kLdsRound - 1) : int(a.sq));
             r.g = uint32_t(top) - r.lo + 1;
