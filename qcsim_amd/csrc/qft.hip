@@ -305,8 +305,13 @@ __global__ __launch_bounds__(256) void k_qft_gpair(cv<T>* __restrict__ psi, uint
             if constexpr (G == 6) __builtin_amdgcn_sched_barrier(0);
         }
     }
+    // store addresses from a copy of p0 the compiler cannot see through, so
+    // that it does not keep all 2^(G-1) 64-bit load addresses live across
+    // the transform
+    cv<T>* sp = p0;
+    asm volatile("" : "+v"(sp));
 #pragma unroll
-    for (int c = 0; c < H; ++c) stamp<T, true>(p0 + c * stride, scaled<T>(x[c], hscale<T>(G)));
+    for (int c = 0; c < H; ++c) stamp<T, true>(sp + c * stride, scaled<T>(x[c], hscale<T>(G)));
 }
 
 // LDS position of chunk element i: bits 0..3 XOR bits 4..7, so that the
