@@ -209,8 +209,10 @@ __global__ __launch_bounds__(256) void k_qft_group(cv<T>* __restrict__ psi, uint
             __builtin_amdgcn_sched_barrier(0);
         }
     }
+    cv<T>* sp = p0;  // store addresses from an opaque copy (see k_qft_gpair)
+    asm volatile("" : "+v"(sp));
 #pragma unroll
-    for (int c = 0; c < D; ++c) stamp<T, true>(p0 + c * stride, scaled<T>(x[c], hscale<T>(G)));
+    for (int c = 0; c < D; ++c) stamp<T, true>(sp + c * stride, scaled<T>(x[c], hscale<T>(G)));
 }
 
 // G = 5 or 6 targets per pass with each 2^G-amplitude orbit split over a
