@@ -790,7 +790,7 @@ static int qft_fused(qcs_register* r, uint32_t sq, uint32_t eq, int do_swap, int
     const bool use_lds = sq < clog && (eq - sq + 1) >= 2;
     const uint32_t lds_hi = use_lds ? (eq < clog - 1 ? eq : clog - 1) : sq - 1;
     // register groups over (lds_hi, eq], in blocks of <= gmax from the top
-    // (QCS_QFT_GROUP overrides the widest group, 1..5)
+    // (QCS_QFT_GROUP overrides the widest group, 1..6)
     static const int gmax = [] {
         const char* v = std::getenv("QCS_QFT_GROUP");
         const int g = v ? std::atoi(v) : kGroupMax;
