@@ -85,7 +85,7 @@ __host__ __device__ __forceinline__ uint64_t arena_xor(uint64_t i) {
 }
 constexpr uint32_t kMixChunks = 4;  // 256-orbit chunks per mixer unit
 constexpr uint32_t kTileRun = 8;    // tiles per tile unit (one instance) with the cost diagonal
-constexpr uint32_t kTileRunRx = 32; // tiles per Rx-only tile unit (no per-tile terms to stage)
+constexpr uint32_t kTileRunRx = 64; // tiles per Rx-only tile unit (no per-tile terms to stage)
 
 // Instance parameters staged in LDS by every unit.
 struct IsingLds {
