@@ -83,7 +83,7 @@ constexpr uint32_t kMaxSites = 64, kMaxEdges = 256;
 __host__ __device__ __forceinline__ uint64_t arena_xor(uint64_t i) {
     return (((i >> 18) ^ (i >> 24)) & 63u) << 4;
 }
-constexpr uint32_t kMixChunks = 4;  // 256-orbit chunks per mixer unit
+constexpr uint32_t kMixChunks = 8;  // 256-orbit chunks per mixer unit
 constexpr uint32_t kTileRun = 8;    // tiles per tile unit (one instance) with the cost diagonal
 constexpr uint32_t kTileRunRx = 64; // tiles per Rx-only tile unit (no per-tile terms to stage)
 
